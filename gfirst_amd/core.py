@@ -1,0 +1,336 @@
+"""Python mirror of the reference's operator interface over the gfr_* C ABI.
+
+Each method of Context corresponds to one host wrapper of the reference (the
+C function it calls cites the reference file:line). Arguments are DeviceArray
+(device-resident) or numpy arrays (uploaded/downloaded around the call).
+Errors raise GfrError with the library's message; nothing falls back to the
+CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import GfrError, GfrReconParams, lib
+
+
+def _check(ctx, rc):
+    if rc < 0:
+        msg = lib().gfr_last_error(ctx).decode() if ctx else ""
+        raise GfrError(rc, msg)
+    return rc
+
+
+class DeviceArray:
+    """A device allocation owned by a Context (gfr_alloc/gfr_free)."""
+
+    def __init__(self, ctx, n, dtype=np.float32):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(ctx.h, lib().gfr_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, self.dtype).reshape(-1)
+        if a.size != self.n:
+            raise ValueError("size mismatch: %d vs %d" % (a.size, self.n))
+        _check(self.ctx.h, lib().gfr_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, self.nbytes,
+                                               None))
+        return self
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        _check(self.ctx.h, lib().gfr_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr,
+                                               self.nbytes, None))
+        return out
+
+    def zero(self):
+        _check(self.ctx.h, lib().gfr_zero(self.ctx.h, self.ptr, self.nbytes, None))
+        return self
+
+    def free(self):
+        if self.ptr:
+            lib().gfr_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device context (replaces the reference's implicit CUDA state)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        h = C.c_void_p()
+        rc = lib().gfr_ctx_create(device, hbm_budget, C.byref(h))
+        if rc != 0:
+            raise GfrError(rc, "gfr_ctx_create(device=%d) failed" % device)
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            lib().gfr_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- memory ---------------------------------------------------------------
+    def array(self, n, dtype=np.float32):
+        return DeviceArray(self, n, dtype)
+
+    def to_device(self, arr, dtype=np.float32):
+        a = np.ascontiguousarray(arr, dtype).reshape(-1)
+        return DeviceArray(self, a.size, dtype).upload(a)
+
+    def sync(self, stream=None):
+        _check(self.h, lib().gfr_sync(self.h, stream))
+
+    def bytes_in_use(self):
+        return lib().gfr_bytes_in_use(self.h)
+
+    def _dev(self, x, n=None, dtype=np.float32):
+        if isinstance(x, DeviceArray):
+            return x, False
+        if x is None:
+            return None, False
+        return self.to_device(x, dtype), True
+
+    # -- operators (one per reference wrapper) ---------------------------------
+    def fill_cylinder(self, g, val, out=None):
+        """CreateInitialCylinder (GFIRST_CPU_Kernels.cpp:35-56)."""
+        o = out if out is not None else self.array(g.nvox)
+        _check(self.h, lib().gfr_fill_cylinder(self.h, C.byref(g.c()), val, o.ptr, None))
+        return o if out is not None else o.download()
+
+    def blur_xy(self, g, img, psf, out=None):
+        """convolutionXY + calcGaussianCoefficients (GFIRST_GPU_Kernels.cu:322-364)."""
+        d, tmp = self._dev(img)
+        o = out if out is not None else self.array(g.nvox)
+        _check(self.h, lib().gfr_blur_xy(self.h, C.byref(g.c()), psf, d.ptr, o.ptr, None))
+        return o if out is not None else o.download()
+
+    def forward_project(self, g, img, subset=0, nsub=1, out=None):
+        """FORWPROJ over the subset's views + scatter (GFIRST_GPU_Kernels.cu:40-107,
+        GFIRST.cu:346-363). Unwritten bins are zero when out is None."""
+        d, _ = self._dev(img)
+        o = out if out is not None else self.array(g.ndata).zero()
+        _check(self.h, lib().gfr_forward_project(self.h, C.byref(g.c()), d.ptr, o.ptr, subset,
+                                                 nsub, None))
+        return o if out is not None else o.download()
+
+    def calib_mask(self, g, proj_blank, norm, out=None):
+        """Mask of GFIRST.cu:453-458 (1 valid, 0 gap)."""
+        pb, _ = self._dev(proj_blank)
+        nm, _ = self._dev(norm)
+        o = out if out is not None else self.array(g.ndata, np.uint8)
+        _check(self.h, lib().gfr_calib_mask(self.h, C.byref(g.c()), pb.ptr, nm.ptr, o.ptr, None))
+        return o if out is not None else o.download()
+
+    def ratio(self, g, proj, y, bg, subset=0, nsub=1, out=None):
+        """Ratio of GFIRST.cu:413-430 (y, bg already x RES)."""
+        p, _ = self._dev(proj)
+        yy, _ = self._dev(y)
+        bb, _ = self._dev(bg)
+        o = out if out is not None else self.array(g.ndata).zero()
+        _check(self.h, lib().gfr_ratio(self.h, C.byref(g.c()), p.ptr, yy.ptr,
+                                       bb.ptr if bb else None, o.ptr, subset, nsub, None))
+        return o if out is not None else o.download()
+
+    def forward_ratio(self, g, img, y, bg, subset=0, nsub=1, out=None, proj_out=None):
+        """Fused forward projection + ratio for the subset's bins."""
+        d, _ = self._dev(img)
+        yy, _ = self._dev(y)
+        bb, _ = self._dev(bg)
+        o = out if out is not None else self.array(g.ndata).zero()
+        _check(self.h, lib().gfr_forward_ratio(self.h, C.byref(g.c()), d.ptr, yy.ptr,
+                                               bb.ptr if bb else None, o.ptr,
+                                               proj_out.ptr if proj_out is not None else None,
+                                               subset, nsub, None))
+        return o if out is not None else o.download()
+
+    def back_project(self, g, corr, mask, subset=0, nsub=1, numer=None, denom=None):
+        """Per-view BACKWPROJ loop of GFIRST.cu:436-469 (GFIRST_GPU_Kernels.cu:109-171)."""
+        c, _ = self._dev(corr)
+        m, _ = self._dev(mask, dtype=np.uint8)
+        n = numer if numer is not None else self.array(g.nvox)
+        d = denom if denom is not None else self.array(g.nvox)
+        _check(self.h, lib().gfr_back_project(self.h, C.byref(g.c()), c.ptr, m.ptr, n.ptr, d.ptr,
+                                              subset, nsub, None))
+        if numer is not None:
+            return n, d
+        return n.download(), d.download()
+
+    def update(self, g, x, numer, denom, psf_bw, work=None):
+        """Image update of GFIRST.cu:482-502 (in place on a DeviceArray x)."""
+        xd, copied = self._dev(x)
+        n, _ = self._dev(numer)
+        d, _ = self._dev(denom)
+        w = work if work is not None else self.array(g.nvox)
+        _check(self.h, lib().gfr_update(self.h, C.byref(g.c()), psf_bw, n.ptr, d.ptr, xd.ptr,
+                                        w.ptr, None))
+        return xd.download() if copied else xd
+
+    def jbf3d(self, g, inp, guide, max_ref, r=4, sigma3d=0.75, sigmai=0.025, out=None):
+        """jointbilateralfilter3D (GFIRST_GPU_Kernels.cu:508-544)."""
+        i, _ = self._dev(inp)
+        gd, _ = self._dev(guide)
+        o = out if out is not None else self.array(g.nvox)
+        _check(self.h, lib().gfr_jbf3d(self.h, C.byref(g.c()), i.ptr, gd.ptr, max_ref, r,
+                                       sigma3d, sigmai, o.ptr, None))
+        return o if out is not None else o.download()
+
+    def blend(self, g, x, filt, coef=0.1):
+        """x = (1-coef) x + coef filt (GFIRST.cu:524), in place on a DeviceArray."""
+        xd, copied = self._dev(x)
+        f, _ = self._dev(filt)
+        _check(self.h, lib().gfr_blend(self.h, C.byref(g.c()), coef, f.ptr, xd.ptr, None))
+        return xd.download() if copied else xd
+
+    def image_max(self, g, x):
+        xd, _ = self._dev(x)
+        m = C.c_float()
+        _check(self.h, lib().gfr_image_max(self.h, C.byref(g.c()), xd.ptr, C.byref(m), None))
+        return m.value
+
+    def vlor(self, g, records, yT_scaled):
+        return Vlor(self, g, records, yT_scaled)
+
+    def recon(self, g, params, d2, t2, norm, records=None):
+        return Recon(self, g, params, d2, t2, norm, records)
+
+
+class Vlor:
+    """VLOR soft-assignment state (GFIRST.cu:229-247, 373-388)."""
+
+    def __init__(self, ctx, g, records, yT_scaled):
+        self.ctx, self.g = ctx, g
+        rec = np.ascontiguousarray(records, np.int32).reshape(-1)
+        y, _ = ctx._dev(yT_scaled)
+        h = C.c_void_p()
+        _check(ctx.h, lib().gfr_vlor_create(ctx.h, C.byref(g.c()), rec.ctypes.data,
+                                            len(rec) // 2, y.ptr, C.byref(h)))
+        self.h = h.value
+
+    def assign(self, proj, subset=0, nsub=1, out=None):
+        p, _ = self.ctx._dev(proj)
+        o = out if out is not None else self.ctx.array(self.g.ndata).zero()
+        _check(self.ctx.h, lib().gfr_vlor_assign(self.ctx.h, self.h, p.ptr, o.ptr, subset, nsub,
+                                                 None))
+        return o if out is not None else o.download()
+
+    def close(self):
+        if self.h:
+            lib().gfr_vlor_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def recon_params(mode=1, niter=1, nsub=1, psf_fw=1.5, psf_bw=1.5, jbf_r=4, jbf_sigma3d=0.75,
+                 jbf_sigmai=0.025, jbf_coef=0.1, storage=_lib.STORE_F32):
+    return GfrReconParams(mode, niter, nsub, psf_fw, psf_bw, jbf_r, jbf_sigma3d, jbf_sigmai,
+                          jbf_coef, storage)
+
+
+def footprint(g, params):
+    return lib().gfr_recon_footprint(C.byref(g.c()), C.byref(params))
+
+
+class Recon:
+    """Device-resident reconstruction session (osem3d, GFIRST.cu:44-575)."""
+
+    def __init__(self, ctx, g, params, d2, t2, norm, records=None):
+        self.ctx, self.g, self.params = ctx, g, params
+        nd = g.ndata
+        d2 = np.ascontiguousarray(d2, np.float32).reshape(-1)
+        norm = np.ascontiguousarray(norm, np.float32).reshape(-1)
+        if d2.size != 2 * nd or norm.size < nd:
+            raise ValueError("d2 must hold 2*NDATA and norm NDATA floats")
+        t2p = None
+        if t2 is not None:
+            t2 = np.ascontiguousarray(t2, np.float32).reshape(-1)
+            if t2.size != 2 * nd:
+                raise ValueError("t2 must hold 2*NDATA floats")
+            t2p = t2.ctypes.data
+        rec = None
+        nrec = 0
+        if records is not None:
+            rec = np.ascontiguousarray(records, np.int32).reshape(-1)
+            nrec = len(rec) // 2
+        h = C.c_void_p()
+        _check(ctx.h, lib().gfr_recon_create(ctx.h, C.byref(g.c()), C.byref(params),
+                                             d2.ctypes.data, t2p, norm.ctypes.data,
+                                             rec.ctypes.data if rec is not None else None, nrec,
+                                             C.byref(h)))
+        self.h = h.value
+
+    def step(self):
+        return _check(self.ctx.h, lib().gfr_recon_step(self.h))
+
+    def run(self):
+        _check(self.ctx.h, lib().gfr_recon_run(self.h))
+
+    def cursor(self):
+        it, ty, s, n = C.c_int(), C.c_int(), C.c_int(), C.c_longlong()
+        done = lib().gfr_recon_cursor(self.h, C.byref(it), C.byref(ty), C.byref(s), C.byref(n))
+        return dict(iter=it.value, type=ty.value, subset=s.value, passes=n.value, done=bool(done))
+
+    def get(self, what):
+        n = self.g.nvox if what in (_lib.OUT_XD, _lib.OUT_XT) else self.g.ndata
+        out = np.empty(n, np.float32)
+        _check(self.ctx.h, lib().gfr_recon_get(self.h, what, out.ctypes.data))
+        return out
+
+    def loglik(self):
+        v = C.c_double()
+        _check(self.ctx.h, lib().gfr_recon_loglik(self.h, C.byref(v)))
+        return v.value
+
+    def profile(self, enable=True):
+        _check(self.ctx.h, lib().gfr_recon_profile(self.h, 1 if enable else 0))
+
+    def phase_times(self):
+        """{phase: (total ms, passes)} since the last call (HIP events)."""
+        ms = np.zeros(len(_lib.PHASES), np.float64)
+        n = np.zeros(len(_lib.PHASES), np.int64)
+        _check(self.ctx.h, lib().gfr_recon_phase_times(self.h, ms.ctypes.data, n.ctypes.data))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(_lib.PHASES)}
+
+    @property
+    def stream(self):
+        return lib().gfr_recon_stream(self.h)
+
+    def sync(self):
+        self.ctx.sync(self.stream)
+
+    def close(self):
+        if self.h:
+            lib().gfr_recon_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,497 @@
+// gfr_api.cpp — context, memory and per-operation entry points of the C ABI.
+// Each entry validates its arguments, resolves the geometry's device tables
+// (cached per context) and enqueues kernels on the given stream; nothing here
+// synchronises except the explicitly synchronous calls.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "gfr_internal.h"
+
+namespace gfr {
+
+int set_err(gfr_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_err(gfr_ctx* ctx, hipError_t e, const char* where) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), "%s: %s", where, hipGetErrorString(e));
+  return set_err(ctx, e == hipErrorOutOfMemory ? GFR_ENOMEM : GFR_EHIP, buf);
+}
+
+hipStream_t stream_of(gfr_ctx* ctx, void* s) { return s ? (hipStream_t)s : ctx->stream; }
+
+GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g) {
+  if (!g) {
+    set_err(ctx, GFR_EINVAL, "geometry is NULL");
+    return nullptr;
+  }
+  std::string key((const char*)g, sizeof(gfr_geometry));
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  auto it = ctx->geoms.find(key);
+  if (it != ctx->geoms.end()) return it->second;
+  GeomDev* gd = new GeomDev();
+  std::string err;
+  if (derive_geometry(g, &gd->host, &err) != GFR_OK) {
+    set_err(ctx, GFR_EINVAL, err);
+    delete gd;
+    return nullptr;
+  }
+  const GeomHost& h = gd->host;
+  const size_t n_ang = (size_t)g->nang, n_seg = (size_t)g->nseg, n_sino = h.sino.size();
+  const size_t off_cos = n_ang * sizeof(float);
+  const size_t off_seg = ((2 * n_ang * sizeof(float)) + 15) & ~size_t(15);
+  const size_t off_sino = (off_seg + n_seg * sizeof(SegEntry) + 15) & ~size_t(15);
+  const size_t total = off_sino + n_sino * sizeof(int2);
+  std::vector<char> blob(total, 0);
+  memcpy(blob.data(), h.sin_t.data(), n_ang * sizeof(float));
+  memcpy(blob.data() + off_cos, h.cos_t.data(), n_ang * sizeof(float));
+  memcpy(blob.data() + off_seg, h.seg.data(), n_seg * sizeof(SegEntry));
+  memcpy(blob.data() + off_sino, h.sino.data(), n_sino * sizeof(int2));
+  hipError_t e = hipMalloc(&gd->block, total);
+  if (e == hipSuccess) e = hipMemcpy(gd->block, blob.data(), total, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hip_err(ctx, e, "geometry tables");
+    if (gd->block) (void)hipFree(gd->block);
+    delete gd;
+    return nullptr;
+  }
+  DevGeom& d = gd->dev;
+  d.res = g->res;
+  d.nzs = g->nzs;
+  d.nrad = g->nrad;
+  d.nang = g->nang;
+  d.nseg = g->nseg;
+  d.nsinogs = h.info.nsinogs;
+  d.l0 = h.info.lambda0;
+  d.l1 = h.info.lambda1;
+  d.RADM = h.RADM;
+  d.RESM = h.RESM;
+  d.fxf = h.fxf;
+  d.fxb = h.fxb;
+  d.ndata = h.info.ndata;
+  d.nvox = h.info.nvox;
+  char* base = (char*)gd->block;
+  d.sin_t = (const float*)base;
+  d.cos_t = (const float*)(base + off_cos);
+  d.seg = (const SegEntry*)(base + off_seg);
+  d.sino = (const int2*)(base + off_sino);
+  ctx->geoms[key] = gd;
+  return gd;
+}
+
+}  // namespace gfr
+
+using namespace gfr;
+
+namespace {
+
+bool bad_subset(gfr_ctx* ctx, const GeomDev* gd, int subset, int nsub) {
+  if (nsub < 1 || nsub > gd->host.g.nang || subset < 0 || subset >= nsub) {
+    set_err(ctx, GFR_EINVAL, "subset out of range (need 0 <= subset < nsub <= nang)");
+    return true;
+  }
+  return false;
+}
+
+#define GFR_CHECK_CTX(ctx) \
+  if (!(ctx)) return GFR_EINVAL
+#define GFR_GEOM(ctx, g, gd)             \
+  GeomDev* gd = geom_for((ctx), (g));    \
+  if (!gd) return GFR_EINVAL
+#define GFR_NONNULL(ctx, p, name) \
+  if (!(p)) return set_err((ctx), GFR_EINVAL, name " is NULL")
+
+}  // namespace
+
+extern "C" {
+
+int gfr_ctx_create(int device, size_t hbm_budget, gfr_ctx** out) {
+  if (!out) return GFR_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GFR_EHIP;
+  if (device < 0 || device >= n) return GFR_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return GFR_EHIP;
+  gfr_ctx* c = new gfr_ctx();
+  c->device = device;
+  c->budget = hbm_budget;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return GFR_EHIP;
+  }
+  if (hipMalloc(&c->d_scratch_u32, 256 * sizeof(unsigned int)) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GFR_ENOMEM;
+  }
+  *out = c;
+  return GFR_OK;
+}
+
+int gfr_ctx_destroy(gfr_ctx* ctx) {
+  if (!ctx) return GFR_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->geoms) {
+    if (kv.second->block) (void)hipFree(kv.second->block);
+    delete kv.second;
+  }
+  for (auto& kv : ctx->allocs) (void)hipFree(kv.first);
+  (void)hipFree(ctx->d_scratch_u32);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return GFR_OK;
+}
+
+const char* gfr_last_error(gfr_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+void* gfr_ctx_stream(gfr_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int gfr_sync(gfr_ctx* ctx, void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_HIP(ctx, hipStreamSynchronize(stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_alloc(gfr_ctx* ctx, size_t bytes, void** dptr) {
+  GFR_CHECK_CTX(ctx);
+  GFR_NONNULL(ctx, dptr, "dptr");
+  *dptr = nullptr;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (ctx->budget && ctx->in_use + bytes > ctx->budget)
+    return set_err(ctx, GFR_ENOMEM, "allocation exceeds the context's HBM budget");
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes ? bytes : 4);
+  if (e != hipSuccess) return hip_err(ctx, e, "hipMalloc");
+  ctx->allocs[p] = bytes;
+  ctx->in_use += bytes;
+  *dptr = p;
+  return GFR_OK;
+}
+
+int gfr_free(gfr_ctx* ctx, void* dptr) {
+  GFR_CHECK_CTX(ctx);
+  if (!dptr) return GFR_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  auto it = ctx->allocs.find(dptr);
+  if (it == ctx->allocs.end()) return set_err(ctx, GFR_EINVAL, "gfr_free: unknown pointer");
+  ctx->in_use -= it->second;
+  ctx->allocs.erase(it);
+  GFR_HIP(ctx, hipFree(dptr));
+  return GFR_OK;
+}
+
+size_t gfr_bytes_in_use(gfr_ctx* ctx) { return ctx ? ctx->in_use : 0; }
+
+int gfr_memcpy_h2d(gfr_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+  GFR_CHECK_CTX(ctx);
+  if (!bytes) return GFR_OK;
+  GFR_NONNULL(ctx, dst, "dst");
+  GFR_NONNULL(ctx, src, "src");
+  hipStream_t st = stream_of(ctx, stream);
+  GFR_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+  GFR_HIP(ctx, hipStreamSynchronize(st));
+  return GFR_OK;
+}
+
+int gfr_memcpy_d2h(gfr_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+  GFR_CHECK_CTX(ctx);
+  if (!bytes) return GFR_OK;
+  GFR_NONNULL(ctx, dst, "dst");
+  GFR_NONNULL(ctx, src, "src");
+  hipStream_t st = stream_of(ctx, stream);
+  GFR_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+  GFR_HIP(ctx, hipStreamSynchronize(st));
+  return GFR_OK;
+}
+
+int gfr_zero(gfr_ctx* ctx, void* dptr, size_t bytes, void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_NONNULL(ctx, dptr, "dptr");
+  GFR_HIP(ctx, launch_zero(dptr, bytes, stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_fill_cylinder(gfr_ctx* ctx, const gfr_geometry* g, float val, float* d_img,
+                      void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_img, "d_img");
+  GFR_HIP(ctx, launch_fill_cylinder(gd->dev, val, d_img, stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_blur_xy(gfr_ctx* ctx, const gfr_geometry* g, float psf, const float* d_in, float* d_out,
+                void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_in, "d_in");
+  GFR_NONNULL(ctx, d_out, "d_out");
+  if (d_in == d_out) return set_err(ctx, GFR_EINVAL, "gfr_blur_xy: d_in == d_out");
+  Taps9 t;
+  psf_taps(psf, t.g);
+  GFR_HIP(ctx, launch_blur_xy(gd->dev, t, d_in, d_out, stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_forward_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_img, float* d_sino,
+                        int subset, int nsub, void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_img, "d_img");
+  GFR_NONNULL(ctx, d_sino, "d_sino");
+  if (bad_subset(ctx, gd, subset, nsub)) return GFR_EINVAL;
+  GFR_HIP(ctx, launch_forward(gd->dev, d_img, d_sino, nullptr, nullptr, GFR_STORE_F32, 1.f,
+                              nullptr, nullptr, subset, nsub, stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_calib_mask(gfr_ctx* ctx, const gfr_geometry* g, const float* d_proj_blank,
+                   const float* d_norm, uint8_t* d_mask, void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_proj_blank, "d_proj_blank");
+  GFR_NONNULL(ctx, d_norm, "d_norm");
+  GFR_NONNULL(ctx, d_mask, "d_mask");
+  GFR_HIP(ctx, launch_calib_mask(gd->dev, d_proj_blank, d_norm, d_mask, stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_ratio(gfr_ctx* ctx, const gfr_geometry* g, const float* d_proj, const float* d_y,
+              const float* d_bg, float* d_corr, int subset, int nsub, void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_proj, "d_proj");
+  GFR_NONNULL(ctx, d_y, "d_y");
+  GFR_NONNULL(ctx, d_corr, "d_corr");
+  if (bad_subset(ctx, gd, subset, nsub)) return GFR_EINVAL;
+  GFR_HIP(ctx, launch_ratio(gd->dev, d_proj, d_y, d_bg, GFR_STORE_F32, 1.f, d_corr, subset, nsub,
+                            stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_forward_ratio(gfr_ctx* ctx, const gfr_geometry* g, const float* d_img, const float* d_y,
+                      const float* d_bg, float* d_corr, float* d_proj, int subset, int nsub,
+                      void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_img, "d_img");
+  GFR_NONNULL(ctx, d_y, "d_y");
+  GFR_NONNULL(ctx, d_corr, "d_corr");
+  if (bad_subset(ctx, gd, subset, nsub)) return GFR_EINVAL;
+  GFR_HIP(ctx, launch_forward(gd->dev, d_img, d_proj, d_y, d_bg, GFR_STORE_F32, 1.f, d_corr,
+                              nullptr, subset, nsub, stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_back_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_corr,
+                     const uint8_t* d_mask, float* d_numer, float* d_denom, int subset, int nsub,
+                     void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_corr, "d_corr");
+  GFR_NONNULL(ctx, d_mask, "d_mask");
+  GFR_NONNULL(ctx, d_numer, "d_numer");
+  GFR_NONNULL(ctx, d_denom, "d_denom");
+  if (bad_subset(ctx, gd, subset, nsub)) return GFR_EINVAL;
+  GFR_HIP(ctx, launch_back(gd->dev, d_corr, d_mask, d_numer, d_denom, subset, nsub,
+                           stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_update(gfr_ctx* ctx, const gfr_geometry* g, float psf_bw, const float* d_numer,
+               const float* d_denom, float* d_x, float* d_work, void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_numer, "d_numer");
+  GFR_NONNULL(ctx, d_denom, "d_denom");
+  GFR_NONNULL(ctx, d_x, "d_x");
+  GFR_NONNULL(ctx, d_work, "d_work");
+  if (g->nzs < 3) return set_err(ctx, GFR_EINVAL, "gfr_update: nzs must be >= 3");
+  Taps9 t9;
+  Taps5 t5;
+  psf_taps(psf_bw, t9.g);
+  gz_taps(t5.g);
+  GFR_HIP(ctx, launch_update(gd->dev, t9, t5, d_numer, d_denom, d_x, d_work,
+                             stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_jbf3d(gfr_ctx* ctx, const gfr_geometry* g, const float* d_in, const float* d_guide,
+              float max_ref, int r, float sigma3d, float sigmai, float* d_out, void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_in, "d_in");
+  GFR_NONNULL(ctx, d_guide, "d_guide");
+  GFR_NONNULL(ctx, d_out, "d_out");
+  if (r < 0 || r > 8) return set_err(ctx, GFR_EINVAL, "gfr_jbf3d: radius must be in [0, 8]");
+  GFR_HIP(ctx, launch_jbf(gd->dev, d_in, d_guide, nullptr, max_ref, r, sigma3d, sigmai, d_out,
+                          stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_blend(gfr_ctx* ctx, const gfr_geometry* g, float coef, const float* d_filt, float* d_x,
+              void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_filt, "d_filt");
+  GFR_NONNULL(ctx, d_x, "d_x");
+  GFR_HIP(ctx, launch_blend(gd->dev, coef, d_filt, d_x, stream_of(ctx, stream)));
+  return GFR_OK;
+}
+
+int gfr_image_max(gfr_ctx* ctx, const gfr_geometry* g, const float* d_x, float* h_max,
+                  void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_x, "d_x");
+  GFR_NONNULL(ctx, h_max, "h_max");
+  hipStream_t st = stream_of(ctx, stream);
+  GFR_HIP(ctx, launch_max(gd->dev, d_x, ctx->d_scratch_u32, st));
+  unsigned int bits = 0;
+  GFR_HIP(ctx, hipMemcpyAsync(&bits, ctx->d_scratch_u32, 4, hipMemcpyDeviceToHost, st));
+  GFR_HIP(ctx, hipStreamSynchronize(st));
+  memcpy(h_max, &bits, 4);
+  return GFR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// VLOR: the static part of the soft assignment (valid records, BLANKT and the
+// bin -> record lists) is built once; each pass recomputes 1/(s1+s2) per
+// record and sums each touched bin's contributions in record order.
+// ---------------------------------------------------------------------------
+int gfr_vlor_create(gfr_ctx* ctx, const gfr_geometry* g, const int32_t* rec, long long nrec,
+                    const float* d_yT, gfr_vlor** out) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, out, "out");
+  GFR_NONNULL(ctx, d_yT, "d_yT");
+  if (nrec < 0 || (nrec > 0 && !rec)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: records");
+  if (nrec > (1LL << 30)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: too many records");
+  *out = nullptr;
+  const long long nd = gd->host.info.ndata;
+  // Valid records (GFIRST.cu:235-237): 1-based indices, 0-based i must satisfy
+  // 0 < i < NDATA for both LORs.
+  std::vector<int> i1, i2;
+  i1.reserve(nrec);
+  i2.reserve(nrec);
+  std::vector<float> v0(nd, 0.f);
+  for (long long e = 0; e < nrec; e++) {
+    const long long a = (long long)rec[2 * e] - 1, b = (long long)rec[2 * e + 1] - 1;
+    if (a <= 0 || b <= 0 || a >= nd || b >= nd) continue;
+    i1.push_back((int)a);
+    i2.push_back((int)b);
+    v0[a] += 1.f;  // datos_v0[i1]++ (GFIRST.cu:238-239)
+    v0[b] += 1.f;
+  }
+  const long long nev = (long long)i1.size();
+  // Bin -> record lists in record order (stable counting sort over touched bins).
+  std::vector<std::pair<long long, int>> ent;
+  ent.reserve(2 * nev);
+  for (long long e = 0; e < nev; e++) {
+    ent.emplace_back(i1[e], (int)e);
+    ent.emplace_back(i2[e], (int)e);
+  }
+  std::stable_sort(ent.begin(), ent.end(),
+                   [](const std::pair<long long, int>& x, const std::pair<long long, int>& y) {
+                     return x.first < y.first;
+                   });
+  std::vector<long long> bins;
+  std::vector<int> start, evs;
+  evs.reserve(ent.size());
+  for (size_t k = 0; k < ent.size(); k++) {
+    if (k == 0 || ent[k].first != ent[k - 1].first) {
+      bins.push_back(ent[k].first);
+      start.push_back((int)k);
+    }
+    evs.push_back(ent[k].second);
+  }
+  start.push_back((int)ent.size());
+  gfr_vlor* v = new gfr_vlor();
+  v->ctx = ctx;
+  v->g = *g;
+  v->nev = nev;
+  v->nbins = (long long)bins.size();
+  hipStream_t st = ctx->stream;
+  auto dalloc = [&](void** p, size_t b) -> hipError_t {
+    v->bytes += b;
+    return hipMalloc(p, b ? b : 4);
+  };
+  float* d_v0 = nullptr;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = dalloc((void**)&v->d_i1, sizeof(int) * nev);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_i2, sizeof(int) * nev);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_blankt, sizeof(float) * nd);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_sinv, sizeof(float) * nev);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_valor, sizeof(float) * nev);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_bins, sizeof(long long) * bins.size());
+  if (e == hipSuccess) e = dalloc((void**)&v->d_start, sizeof(int) * start.size());
+  if (e == hipSuccess) e = dalloc((void**)&v->d_ev, sizeof(int) * evs.size());
+  if (e == hipSuccess) e = hipMalloc((void**)&d_v0, sizeof(float) * nd);
+  if (e == hipSuccess && nev) {
+    e = hipMemcpy(v->d_i1, i1.data(), sizeof(int) * nev, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(v->d_i2, i2.data(), sizeof(int) * nev, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(v->d_bins, bins.data(), sizeof(long long) * bins.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(v->d_ev, evs.data(), sizeof(int) * evs.size(), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(v->d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_v0, v0.data(), sizeof(float) * nd, hipMemcpyHostToDevice);
+  // BLANKT = T/(RES*v0) where v0 > 0.01 (GFIRST.cu:244-247); T already x RES.
+  if (e == hipSuccess) e = launch_vlor_blankt(d_yT, d_v0, (float)g->res, v->d_blankt, nd, st);
+  // valor per record is static: BLANKT[i1] + BLANKT[i2].
+  if (e == hipSuccess)
+    e = launch_vlor_event(v->d_i1, v->d_i2, v->d_blankt, v->d_blankt, v->d_sinv, v->d_valor, nev,
+                          1, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (d_v0) (void)hipFree(d_v0);
+  if (e != hipSuccess) {
+    v->ctx = nullptr;  // its bytes were never counted in the context
+    gfr_vlor_destroy(v);
+    return hip_err(ctx, e, "gfr_vlor_create");
+  }
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->in_use += v->bytes;
+  }
+  *out = v;
+  return GFR_OK;
+}
+
+int gfr_vlor_assign(gfr_ctx* ctx, gfr_vlor* v, const float* d_proj, float* d_dv, int subset,
+                    int nsub, void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_NONNULL(ctx, v, "vlor");
+  GFR_NONNULL(ctx, d_proj, "d_proj");
+  GFR_NONNULL(ctx, d_dv, "d_dv");
+  GFR_GEOM(ctx, &v->g, gd);
+  if (bad_subset(ctx, gd, subset, nsub)) return GFR_EINVAL;
+  hipStream_t st = stream_of(ctx, stream);
+  GFR_HIP(ctx, launch_vlor_event(v->d_i1, v->d_i2, v->d_blankt, d_proj, v->d_sinv, v->d_valor,
+                                 v->nev, 0, st));
+  GFR_HIP(ctx, launch_vlor_zero_subset(gd->dev, d_dv, subset, nsub, st));
+  GFR_HIP(ctx, launch_vlor_bins(gd->dev, v->d_bins, v->d_start, v->d_ev, v->nbins, d_proj,
+                                v->d_sinv, v->d_valor, d_dv, subset, nsub, st));
+  return GFR_OK;
+}
+
+int gfr_vlor_destroy(gfr_vlor* v) {
+  if (!v) return GFR_EINVAL;
+  if (v->ctx) {
+    std::lock_guard<std::mutex> lk(v->ctx->mu);
+    if (v->ctx->in_use >= v->bytes) v->ctx->in_use -= v->bytes;
+  }
+  void* ps[] = {v->d_i1, v->d_i2, v->d_blankt, v->d_sinv, v->d_valor, v->d_bins, v->d_start, v->d_ev};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  delete v;
+  return GFR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+// gfr_internal.h — internal types shared by the HIP kernels and the host
+// code of libgfirst.so (not part of the public C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gfirst.h"
+
+namespace gfr {
+
+// Per-segment constants, segment order 0,-1,+1,-2,+2,... (GFIRST.cu:163-168).
+struct SegEntry {
+  int zmin;   // minimum ring difference / first valid plane (GFIRST_GPU_Kernels.cu:54)
+  int nza;    // first sinogram of the segment (nz_acum, GFIRST.cu:133-140)
+  float dzf;  // forward-projector z step per sample (GFIRST_GPU_Kernels.cu:83)
+  float fzb;  // backprojector z tilt (GFIRST_GPU_Kernels.cu:134, via double)
+};
+
+// Geometry as the kernels see it (by value). Pointers are device tables owned
+// by the context, built once per distinct geometry.
+struct DevGeom {
+  int res, nzs, nrad, nang, nseg, nsinogs;
+  int l0, l1;         // sample index range along a LOR
+  float RADM, RESM;   // (NRAD-1)/2, (RES-1)/2
+  float fxf;          // forward: detector pixel -> voxel (px/vx)
+  float fxb;          // backward: voxel -> detector pixel (vx/px)
+  long long ndata, nvox;
+  const float* sin_t;     // [nang]
+  const float* cos_t;     // [nang]
+  const SegEntry* seg;    // [nseg]
+  const int2* sino;       // [nsinogs]: (segment index, absolute plane)
+};
+
+struct Taps9 { float g[9]; };
+struct Taps5 { float g[5]; };
+
+// Host-side geometry derivation (shared by the API and the kernels' launchers).
+struct GeomHost {
+  gfr_geometry g;
+  gfr_geometry_info info;
+  std::vector<int> zmin, sgn, nza;
+  std::vector<float> sin_t, cos_t;
+  std::vector<SegEntry> seg;
+  std::vector<int2> sino;
+  float RADM, RESM, fxf, fxb;
+};
+
+int derive_geometry(const gfr_geometry* g, GeomHost* out, std::string* err);
+void psf_taps(float psf, float* g9);
+void gz_taps(float* g5);
+
+struct GeomDev {
+  GeomHost host;
+  DevGeom dev;
+  void* block = nullptr;  // single device allocation holding the tables
+};
+
+}  // namespace gfr
+
+struct gfr_ctx {
+  int device = 0;
+  size_t budget = 0;
+  size_t in_use = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::map<std::string, gfr::GeomDev*> geoms;
+  std::map<void*, size_t> allocs;
+  unsigned int* d_scratch_u32 = nullptr;  // 256 words for small reductions
+  std::mutex mu;
+};
+
+namespace gfr {
+
+// Error helpers used by every entry point.
+int set_err(gfr_ctx* ctx, int code, const std::string& msg);
+int hip_err(gfr_ctx* ctx, hipError_t e, const char* where);
+#define GFR_HIP(ctx, call)                                   \
+  do {                                                       \
+    hipError_t _e = (call);                                  \
+    if (_e != hipSuccess) return gfr::hip_err(ctx, _e, #call); \
+  } while (0)
+
+// Resolves (and caches) the device tables of a geometry.
+GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g);
+hipStream_t stream_of(gfr_ctx* ctx, void* s);
+
+// Kernel launchers (gfr_kernels.hip). All return hipError_t of the launch.
+hipError_t launch_fill_cylinder(const DevGeom& G, float val, float* img, hipStream_t st);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);
+hipError_t launch_blur_xy(const DevGeom& G, const Taps9& t, const float* in, float* out,
+                          hipStream_t st);
+hipError_t launch_forward(const DevGeom& G, const float* img, float* proj, const void* y,
+                          const void* bg, int storage, float yscale, float* corr,
+                          double* llh_part, int subset, int nsub, hipStream_t st);
+hipError_t launch_ratio(const DevGeom& G, const float* proj, const float* y, const void* bg,
+                        int storage, float yscale, float* corr, int subset, int nsub,
+                        hipStream_t st);
+hipError_t launch_calib_mask(const DevGeom& G, const float* pb, const float* norm,
+                             uint8_t* mask, hipStream_t st);
+hipError_t launch_back(const DevGeom& G, const float* corr, const uint8_t* mask, float* numer,
+                       float* denom, int subset, int nsub, hipStream_t st);
+hipError_t launch_update(const DevGeom& G, const Taps9& t9, const Taps5& t5, const float* numer,
+                         const float* denom, float* x, float* work, hipStream_t st);
+hipError_t launch_jbf(const DevGeom& G, const float* in, const float* guide,
+                      const float* d_max_ref, float max_ref, int r, float s3d, float sI,
+                      float* out, hipStream_t st);
+hipError_t launch_blend(const DevGeom& G, float coef, const float* filt, float* x,
+                        hipStream_t st);
+hipError_t launch_max(const DevGeom& G, const float* x, unsigned int* d_bits, hipStream_t st);
+hipError_t launch_scale_copy(const float* in, float s, float* out, long long n, hipStream_t st);
+hipError_t launch_convert_store(const float* in, void* out, int storage, long long n,
+                                hipStream_t st);
+hipError_t launch_llh_final(const double* part, int n, double* out, hipStream_t st);
+
+// VLOR.
+hipError_t launch_vlor_blankt(const float* yT, const float* v0, float res, float* blankt,
+                              long long nd, hipStream_t st);
+hipError_t launch_vlor_event(const int* i1, const int* i2, const float* blankt,
+                             const float* proj, float* sinv, float* valor, long long nev,
+                             int want_valor, hipStream_t st);
+hipError_t launch_vlor_zero_subset(const DevGeom& G, float* dv, int subset, int nsub,
+                                   hipStream_t st);
+hipError_t launch_vlor_bins(const DevGeom& G, const long long* bins, const int* start,
+                            const int* ev, long long nbins, const float* proj,
+                            const float* sinv, const float* valor, float* dv, int subset,
+                            int nsub, hipStream_t st);
+
+}  // namespace gfr
+
+struct gfr_vlor {
+  gfr_ctx* ctx = nullptr;
+  gfr_geometry g;
+  long long nev = 0;      // valid records
+  long long nbins = 0;    // touched bins
+  int* d_i1 = nullptr;    // [nev] 0-based
+  int* d_i2 = nullptr;
+  float* d_blankt = nullptr;  // [ndata]
+  float* d_sinv = nullptr;    // [nev]
+  float* d_valor = nullptr;   // [nev]
+  long long* d_bins = nullptr;  // [nbins] bin ids, ascending
+  int* d_start = nullptr;       // [nbins+1] offsets into d_ev
+  int* d_ev = nullptr;          // [2*nev] event ids per bin, record order
+  size_t bytes = 0;
+};

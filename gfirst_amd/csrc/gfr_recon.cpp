@@ -1,0 +1,416 @@
+// gfr_recon.cpp — whole reconstruction as a device-resident session
+// (osem3d, GFIRST.cu:44-575). Data are uploaded once; every array stays in
+// HBM; a step is one (type, subset) pass and enqueues only kernels (no host
+// round trip), so steps of different sessions overlap on their own streams.
+//
+// Loop nest: iter -> type (doubles, then triples) -> subset. Subset s holds
+// the angles theta with theta % S == s; S = 1 is the reference's loop.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "gfr_internal.h"
+
+using namespace gfr;
+
+struct gfr_recon {
+  gfr_ctx* ctx = nullptr;
+  gfr_geometry g{};
+  gfr_recon_params p{};
+  GeomDev* gd = nullptr;
+  hipStream_t st = nullptr;
+  int ntypes = 1;
+  // image-sized
+  float *xD = nullptr, *xT = nullptr, *xb = nullptr, *num = nullptr, *den = nullptr,
+        *work = nullptr, *jo = nullptr;
+  // sinogram-sized
+  float *corr = nullptr, *pb = nullptr, *ptc = nullptr, *dv = nullptr;
+  void *yD = nullptr, *bD = nullptr, *yT = nullptr, *bT = nullptr;
+  uint8_t* mask = nullptr;
+  gfr_vlor* vl = nullptr;
+  double* llh_part = nullptr;
+  double* d_llh = nullptr;
+  unsigned int* d_maxbits = nullptr;
+  int llh_valid = 0;
+  std::vector<void*> owned;
+  size_t bytes = 0;
+  int iter = 1, type = 0, subset = 0;
+  long long passes = 0;
+  bool done = false;
+  // tracing: event pairs per phase, resolved lazily by gfr_recon_phase_times
+  bool prof = false;
+  std::vector<hipEvent_t> ev_free;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  double phase_ms[GFR_NPHASES] = {0};
+  long long phase_n[GFR_NPHASES] = {0};
+};
+
+namespace {
+
+size_t store_bytes(int storage) { return storage == GFR_STORE_F32 ? 4 : 2; }
+
+int llh_blocks(const GeomHost& h, int nsub) {
+  const int nav = (h.g.nang + nsub - 1) / nsub;
+  return ((h.g.nrad + 63) / 64) * ((h.info.nsinogs + 3) / 4) * nav;
+}
+
+struct Plan {
+  size_t img, sino_f32, sino_store, mask, llh;
+  int nimg, nsino_f32, nsino_store;
+};
+
+bool any_positive(const float* v, long long n) {
+  for (long long i = 0; i < n; i++)
+    if (!(v[i] <= 0.f)) return true;  // NaN counts as present
+  return false;
+}
+
+Plan plan_of(const GeomHost& h, const gfr_recon_params* p, bool bgD, bool bgT) {
+  Plan pl{};
+  pl.img = sizeof(float) * (size_t)h.info.nvox;
+  pl.sino_f32 = sizeof(float) * (size_t)h.info.ndata;
+  pl.sino_store = store_bytes(p->storage) * (size_t)h.info.ndata;
+  pl.mask = (size_t)h.info.ndata;
+  pl.llh = sizeof(double) * (size_t)llh_blocks(h, p->nsub) + 64;
+  const bool two = p->mode >= 2;
+  pl.nimg = 6 + (two ? 2 : 0);                      // xD num den work xb (+xT jo)
+  pl.nsino_f32 = 2 + (p->mode == 3 ? 2 : 0);        // corr pb (+ptc dv)
+  pl.nsino_store = 1 + (bgD ? 1 : 0) + (two ? 1 + (bgT ? 1 : 0) : 0);
+  return pl;
+}
+
+size_t plan_bytes(const Plan& pl) {
+  return pl.nimg * pl.img + pl.nsino_f32 * pl.sino_f32 + pl.nsino_store * pl.sino_store +
+         pl.mask + pl.llh;
+}
+
+hipEvent_t take_event(gfr_recon* r) {
+  if (!r->ev_free.empty()) {
+    hipEvent_t e = r->ev_free.back();
+    r->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one phase of a pass with events when tracing is on.
+struct PhaseMark {
+  gfr_recon* r;
+  int phase;
+  hipEvent_t e0 = nullptr;
+  PhaseMark(gfr_recon* rr, int ph) : r(rr), phase(ph) {
+    if (r->prof) {
+      e0 = take_event(r);
+      if (e0) (void)hipEventRecord(e0, r->st);
+    }
+  }
+  void end() {
+    if (!e0) return;
+    hipEvent_t e1 = take_event(r);
+    if (e1 && hipEventRecord(e1, r->st) == hipSuccess) {
+      r->ev_pending.push_back({phase, {e0, e1}});
+    } else {
+      r->ev_free.push_back(e0);
+      if (e1) r->ev_free.push_back(e1);
+    }
+    e0 = nullptr;
+  }
+  ~PhaseMark() { end(); }
+};
+
+int rfail(gfr_recon* r, int code) {
+  gfr_recon_destroy(r);
+  return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+long long gfr_recon_footprint(const gfr_geometry* g, const gfr_recon_params* p) {
+  GeomHost h;
+  if (!p || derive_geometry(g, &h, nullptr) != GFR_OK) return GFR_EINVAL;
+  return (long long)plan_bytes(plan_of(h, p, true, true));
+}
+
+int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params* p,
+                     const float* d2, const float* t2, const float* norm, const int32_t* rec,
+                     long long nrec, gfr_recon** out) {
+  if (!ctx || !out) return GFR_EINVAL;
+  *out = nullptr;
+  if (!p) return set_err(ctx, GFR_EINVAL, "params is NULL");
+  GeomDev* gd = geom_for(ctx, g);
+  if (!gd) return GFR_EINVAL;
+  const GeomHost& h = gd->host;
+  if (p->mode < 1 || p->mode > 3) return set_err(ctx, GFR_EINVAL, "mode must be 1, 2 or 3");
+  if (p->niter < 0 || p->nsub < 1 || p->nsub > g->nang)
+    return set_err(ctx, GFR_EINVAL, "niter >= 0 and 1 <= nsub <= nang required");
+  if (p->storage < GFR_STORE_F32 || p->storage > GFR_STORE_BF16)
+    return set_err(ctx, GFR_EINVAL, "unknown storage");
+  if (g->nzs < 3) return set_err(ctx, GFR_EINVAL, "nzs must be >= 3");
+  if (p->jbf_r < 0 || p->jbf_r > 8) return set_err(ctx, GFR_EINVAL, "jbf_r must be in [0, 8]");
+  if (!d2 || !norm) return set_err(ctx, GFR_EINVAL, "doubles sinogram and norm are required");
+  if (p->mode >= 2 && !t2) return set_err(ctx, GFR_EINVAL, "triples sinogram required");
+  if (p->mode == 3 && nrec > 0 && !rec) return set_err(ctx, GFR_EINVAL, "VLOR records required");
+  const long long nd = h.info.ndata;
+  const bool hasBgD = any_positive(d2 + nd, nd);
+  const bool hasBgT = p->mode >= 2 && any_positive(t2 + nd, nd);
+  const Plan pl = plan_of(h, p, hasBgD, hasBgT);
+  const size_t need = plan_bytes(pl);
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->budget && ctx->in_use + need > ctx->budget)
+      return set_err(ctx, GFR_ENOMEM, "reconstruction exceeds the context's HBM budget");
+    ctx->in_use += need;
+  }
+  gfr_recon* r = new gfr_recon();
+  r->ctx = ctx;
+  r->g = *g;
+  r->p = *p;
+  r->gd = gd;
+  r->bytes = need;
+  r->ntypes = p->mode >= 2 ? 2 : 1;
+  if (hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking) != hipSuccess)
+    return rfail(r, set_err(ctx, GFR_EHIP, "stream create"));
+  auto A = [&](void** ptr, size_t b) -> bool {
+    if (hipMalloc(ptr, b ? b : 4) != hipSuccess) return false;
+    r->owned.push_back(*ptr);
+    return true;
+  };
+  bool ok = A((void**)&r->xD, pl.img) && A((void**)&r->num, pl.img) && A((void**)&r->den, pl.img) &&
+            A((void**)&r->work, pl.img) && A((void**)&r->xb, pl.img) &&
+            A((void**)&r->corr, pl.sino_f32) && A((void**)&r->pb, pl.sino_f32) &&
+            A((void**)&r->mask, pl.mask) && A((void**)&r->yD, pl.sino_store) &&
+            A((void**)&r->llh_part, pl.llh) && A((void**)&r->d_maxbits, 64) &&
+            A((void**)&r->d_llh, 64);
+  if (ok && hasBgD) ok = A(&r->bD, pl.sino_store);
+  if (ok && r->ntypes == 2)
+    ok = A((void**)&r->xT, pl.img) && A((void**)&r->jo, pl.img) && A(&r->yT, pl.sino_store) &&
+         (!hasBgT || A(&r->bT, pl.sino_store));
+  if (ok && p->mode == 3) ok = A((void**)&r->ptc, pl.sino_f32) && A((void**)&r->dv, pl.sino_f32);
+  if (!ok) return rfail(r, set_err(ctx, GFR_ENOMEM, "reconstruction allocation failed"));
+  hipStream_t st = r->st;
+  const DevGeom& G = gd->dev;
+  hipError_t e = hipSuccess;
+  // Upload a host float array into storage format through the corr buffer.
+  auto upload = [&](const float* host, void* dst) -> hipError_t {
+    hipError_t ee = hipMemcpyAsync(r->corr, host, pl.sino_f32, hipMemcpyHostToDevice, st);
+    if (ee == hipSuccess) ee = launch_convert_store(r->corr, dst, p->storage, nd, st);
+    if (ee == hipSuccess) ee = hipStreamSynchronize(st);
+    return ee;
+  };
+  // Mask source: projection of the unblurred uniform cylinder (GFIRST.cu:155-181)
+  // and the calibration mask (GFIRST.cu:453-458) from the normalization.
+  e = launch_fill_cylinder(G, 1.f, r->xb, st);
+  if (e == hipSuccess)
+    e = launch_forward(G, r->xb, r->pb, nullptr, nullptr, GFR_STORE_F32, 1.f, nullptr, nullptr, 0,
+                       1, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(r->corr, norm, pl.sino_f32, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_calib_mask(G, r->pb, r->corr, r->mask, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  // Measured data, stored unscaled; kernels scale by RES on load (GFIRST.cu:224-260).
+  if (e == hipSuccess) e = upload(d2, r->yD);
+  if (e == hipSuccess && hasBgD) e = upload(d2 + nd, r->bD);
+  if (e == hipSuccess && r->ntypes == 2) e = upload(t2, r->yT);
+  if (e == hipSuccess && hasBgT) e = upload(t2 + nd, r->bT);
+  // Initial estimates (GFIRST.cu:282-288).
+  if (e == hipSuccess) e = launch_fill_cylinder(G, 1.f, r->xD, st);
+  if (e == hipSuccess && r->ntypes == 2)
+    e = launch_fill_cylinder(G, p->mode == 1 ? 0.f : 1.f, r->xT, st);
+  if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create"));
+  if (p->mode == 3) {
+    // BLANKT needs T x RES (GFIRST.cu:225-226, 246): scale a float copy.
+    e = hipMemcpyAsync(r->ptc, t2, pl.sino_f32, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_scale_copy(r->ptc, (float)g->res, r->dv, nd, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create vlor"));
+    int rc = gfr_vlor_create(ctx, g, rec, nrec, r->dv, &r->vl);
+    if (rc != GFR_OK) return rfail(r, rc);
+    // Projection cache of the triples estimate (bins outside the current
+    // subset keep their last projection).
+    Taps9 t9;
+    psf_taps(p->psf_fw, t9.g);
+    e = launch_zero(r->dv, pl.sino_f32, st);
+    if (e == hipSuccess) e = launch_blur_xy(G, t9, r->xT, r->xb, st);
+    if (e == hipSuccess)
+      e = launch_forward(G, r->xb, r->ptc, nullptr, nullptr, GFR_STORE_F32, 1.f, nullptr, nullptr,
+                         0, 1, st);
+    if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create vlor cache"));
+  }
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create sync"));
+  r->done = (p->niter == 0);
+  *out = r;
+  return GFR_OK;
+}
+
+int gfr_recon_step(gfr_recon* r) {
+  if (!r) return GFR_EINVAL;
+  if (r->done) return 1;
+  gfr_ctx* ctx = r->ctx;
+  const DevGeom& G = r->gd->dev;
+  const gfr_recon_params& p = r->p;
+  hipStream_t st = r->st;
+  const int ty = r->type, s = r->subset, S = p.nsub;
+  float* x = ty == 0 ? r->xD : r->xT;
+  Taps9 tf, tb;
+  Taps5 tz;
+  psf_taps(p.psf_fw, tf.g);
+  psf_taps(p.psf_bw, tb.g);
+  gz_taps(tz.g);
+  // PSF blur before projection (GFIRST.cu:324-340).
+  {
+    PhaseMark pm(r, GFR_PHASE_BLUR);
+    GFR_HIP(ctx, launch_blur_xy(G, tf, x, r->xb, st));
+  }
+  if (ty == 1 && p.mode == 3) {
+    // Forward projection into the cache, VLOR reassignment of the subset's bins
+    // (GFIRST.cu:373-397), ratio against the pseudo-sinogram x RES.
+    {
+      PhaseMark pm(r, GFR_PHASE_FORWARD);
+      GFR_HIP(ctx, launch_forward(G, r->xb, r->ptc, nullptr, nullptr, GFR_STORE_F32, 1.f, nullptr,
+                                  nullptr, s, S, st));
+    }
+    PhaseMark pm(r, GFR_PHASE_VLOR);
+    int rc = gfr_vlor_assign(ctx, r->vl, r->ptc, r->dv, s, S, st);
+    if (rc != GFR_OK) return rc;
+    GFR_HIP(ctx, launch_ratio(G, r->ptc, r->dv, r->bT, p.storage, (float)r->g.res, r->corr, s, S,
+                              st));
+    r->llh_valid = 0;
+  } else {
+    const void* y = ty == 0 ? r->yD : r->yT;
+    const void* bg = ty == 0 ? r->bD : r->bT;
+    {
+      PhaseMark pm(r, GFR_PHASE_FORWARD);
+      GFR_HIP(ctx, launch_forward(G, r->xb, nullptr, y, bg, p.storage, (float)r->g.res, r->corr,
+                                  r->llh_part, s, S, st));
+    }
+    GFR_HIP(ctx, launch_llh_final(r->llh_part, llh_blocks(r->gd->host, S), r->d_llh, st));
+    r->llh_valid = 1;
+  }
+  // Backprojection of the subset (GFIRST.cu:436-469) and update (:482-502).
+  {
+    PhaseMark pm(r, GFR_PHASE_BACK);
+    GFR_HIP(ctx, launch_back(G, r->corr, r->mask, r->num, r->den, s, S, st));
+  }
+  {
+    PhaseMark pm(r, GFR_PHASE_UPDATE);
+    GFR_HIP(ctx, launch_update(G, tb, tz, r->num, r->den, x, r->work, st));
+  }
+  if (ty == 1) {
+    // Joint bilateral filter of triples guided by doubles + blend (:504-526).
+    PhaseMark pm(r, GFR_PHASE_JBF);
+    GFR_HIP(ctx, launch_max(G, r->xD, r->d_maxbits, st));
+    GFR_HIP(ctx, launch_jbf(G, r->xT, r->xD, (const float*)r->d_maxbits, 0.f, p.jbf_r,
+                            p.jbf_sigma3d, p.jbf_sigmai, r->jo, st));
+    GFR_HIP(ctx, launch_blend(G, p.jbf_coef, r->jo, r->xT, st));
+  }
+  r->passes++;
+  if (++r->subset == S) {
+    r->subset = 0;
+    if (++r->type == r->ntypes) {
+      r->type = 0;
+      if (++r->iter > p.niter) r->done = true;
+    }
+  }
+  return r->done ? 1 : 0;
+}
+
+int gfr_recon_run(gfr_recon* r) {
+  if (!r) return GFR_EINVAL;
+  int rc = 0;
+  while ((rc = gfr_recon_step(r)) == 0) {
+  }
+  if (rc < 0) return rc;
+  GFR_HIP(r->ctx, hipStreamSynchronize(r->st));
+  return GFR_OK;
+}
+
+int gfr_recon_cursor(gfr_recon* r, int* iter, int* type, int* subset, long long* passes) {
+  if (!r) return GFR_EINVAL;
+  if (iter) *iter = r->iter;
+  if (type) *type = r->type;
+  if (subset) *subset = r->subset;
+  if (passes) *passes = r->passes;
+  return r->done ? 1 : 0;
+}
+
+int gfr_recon_get(gfr_recon* r, int what, float* host_out) {
+  if (!r || !host_out) return GFR_EINVAL;
+  const GeomHost& h = r->gd->host;
+  const void* src = nullptr;
+  size_t bytes = 0;
+  switch (what) {
+    case GFR_OUT_XD: src = r->xD; bytes = sizeof(float) * h.info.nvox; break;
+    case GFR_OUT_XT: src = r->xT; bytes = sizeof(float) * h.info.nvox; break;
+    case GFR_OUT_PROJ_BLANK: src = r->pb; bytes = sizeof(float) * h.info.ndata; break;
+    case GFR_OUT_VLOR: src = r->dv; bytes = sizeof(float) * h.info.ndata; break;
+    default: return set_err(r->ctx, GFR_EINVAL, "gfr_recon_get: unknown output");
+  }
+  if (!src) return set_err(r->ctx, GFR_ESTATE, "gfr_recon_get: output not held in this mode");
+  GFR_HIP(r->ctx, hipMemcpyAsync(host_out, src, bytes, hipMemcpyDeviceToHost, r->st));
+  GFR_HIP(r->ctx, hipStreamSynchronize(r->st));
+  return GFR_OK;
+}
+
+int gfr_recon_loglik(gfr_recon* r, double* out) {
+  if (!r || !out) return GFR_EINVAL;
+  if (!r->llh_valid) return set_err(r->ctx, GFR_ESTATE, "no log-likelihood for the last pass");
+  GFR_HIP(r->ctx, hipMemcpyAsync(out, r->d_llh, sizeof(double), hipMemcpyDeviceToHost, r->st));
+  GFR_HIP(r->ctx, hipStreamSynchronize(r->st));
+  return GFR_OK;
+}
+
+void* gfr_recon_stream(gfr_recon* r) { return r ? (void*)r->st : nullptr; }
+
+int gfr_recon_profile(gfr_recon* r, int enable) {
+  if (!r) return GFR_EINVAL;
+  r->prof = enable != 0;
+  return GFR_OK;
+}
+
+int gfr_recon_phase_times(gfr_recon* r, double* ms, long long* counts) {
+  if (!r) return GFR_EINVAL;
+  GFR_HIP(r->ctx, hipStreamSynchronize(r->st));
+  for (auto& pe : r->ev_pending) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, pe.second.first, pe.second.second) == hipSuccess) {
+      r->phase_ms[pe.first] += t;
+      r->phase_n[pe.first] += 1;
+    }
+    r->ev_free.push_back(pe.second.first);
+    r->ev_free.push_back(pe.second.second);
+  }
+  r->ev_pending.clear();
+  for (int i = 0; i < GFR_NPHASES; i++) {
+    if (ms) ms[i] = r->phase_ms[i];
+    if (counts) counts[i] = r->phase_n[i];
+    r->phase_ms[i] = 0.0;
+    r->phase_n[i] = 0;
+  }
+  return GFR_OK;
+}
+
+int gfr_recon_destroy(gfr_recon* r) {
+  if (!r) return GFR_EINVAL;
+  if (r->st) (void)hipStreamSynchronize(r->st);
+  if (r->vl) gfr_vlor_destroy(r->vl);
+  for (auto& pe : r->ev_pending) {
+    (void)hipEventDestroy(pe.second.first);
+    (void)hipEventDestroy(pe.second.second);
+  }
+  for (hipEvent_t e : r->ev_free) (void)hipEventDestroy(e);
+  for (void* q : r->owned) (void)hipFree(q);
+  if (r->st) (void)hipStreamDestroy(r->st);
+  {
+    std::lock_guard<std::mutex> lk(r->ctx->mu);
+    r->ctx->in_use -= r->bytes;
+  }
+  delete r;
+  return GFR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+"""Seeded synthetic inputs for the OSEM path (phantoms, Poisson sinograms).
+
+Everything derives from numpy's PCG64 generator with an explicit seed, so any
+machine with the same numpy regenerates identical inputs.
+"""
+import numpy as np
+
+from .geometry import Geometry
+
+
+def voxel_centres_mm(g: Geometry):
+    """(z, y, x) voxel-centre coordinates in mm; the z voxel is pitch/2
+    (planes index slices directly, GFIRST_GPU_Kernels.cu:88)."""
+    vx = g.fov / g.res
+    vz = g.pitch / 2.0
+    x = (np.arange(g.res) - (g.res - 1) / 2.0) * vx
+    z = (np.arange(g.nzs) - (g.nzs - 1) / 2.0) * vz
+    return z, x, x
+
+
+def _ellipsoid(g, centre, semi, value, img):
+    z, y, x = voxel_centres_mm(g)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    m = (((X - centre[0]) / semi[0]) ** 2 + ((Y - centre[1]) / semi[1]) ** 2 +
+         ((Z - centre[2]) / semi[2]) ** 2) <= 1.0
+    img[m] = value
+    return img
+
+
+def phantom_c0(g: Geometry, seed=0, radius=25.0, sphere_r=4.0, hot=4.0):
+    """Uniform cylinder (radius 25 mm, value 1, all slices) with 3 hot spheres
+    (radius 4 mm, value 4) whose centres are drawn inside r <= 17 mm."""
+    rng = np.random.default_rng(seed)
+    z, y, x = voxel_centres_mm(g)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    img = np.where(X ** 2 + Y ** 2 <= radius ** 2, 1.0, 0.0).astype(np.float32)
+    zr = max(abs(z[0]) - sphere_r, 0.0)
+    for _ in range(3):
+        rr = 17.0 * np.sqrt(rng.uniform())
+        ph = rng.uniform(0, 2 * np.pi)
+        cz = rng.uniform(-zr, zr)
+        m = (X - rr * np.cos(ph)) ** 2 + (Y - rr * np.sin(ph)) ** 2 + (Z - cz) ** 2 <= sphere_r ** 2
+        img[m] = hot
+    return img.reshape(-1)
+
+
+def phantom_mouse(g: Geometry, seed=0, channel=0):
+    """Mouse-like union of ellipsoids: body 25x25x90 mm (full axes), value 1,
+    and 6 organ ellipsoids with values in [0.5, 8] at seeded positions.
+    channel selects an independent organ draw (tracer A = 0, B = 1)."""
+    rng = np.random.default_rng(seed * 7919 + channel)
+    img = np.zeros((g.nzs, g.res, g.res), np.float32)
+    zhalf = min(45.0, (g.nzs - 1) / 2.0 * g.pitch / 2.0)
+    _ellipsoid(g, (0.0, 0.0, 0.0), (12.5, 12.5, zhalf), 1.0, img)
+    vals = rng.uniform(0.5, 8.0, size=6)
+    for k in range(6):
+        semi = rng.uniform(2.0, 5.0, size=3)
+        semi[2] = rng.uniform(3.0, 10.0)
+        r = rng.uniform(0, 12.5 - semi[:2].max())
+        ph = rng.uniform(0, 2 * np.pi)
+        cz = rng.uniform(-(zhalf - semi[2]), zhalf - semi[2])
+        _ellipsoid(g, (r * np.cos(ph), r * np.sin(ph), cz), semi, vals[k], img)
+    return img.reshape(-1)
+
+
+def poisson_sinogram(ybar, total_counts, seed=0):
+    """Scale the expected sinogram to total_counts and draw Poisson counts
+    (int32, as a prompts file the reader takes the integer path for)."""
+    ybar = np.asarray(ybar, np.float64)
+    s = ybar.sum()
+    lam = ybar * (total_counts / s) if s > 0 else ybar
+    rng = np.random.default_rng(seed)
+    return rng.poisson(lam).astype(np.int32)
+
+
+def norm_uniform(ndata, lo=0.8, hi=1.2, seed=0):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(lo, hi, size=ndata).astype(np.float32)
+
+
+def as_file_pair(prompts, background=None):
+    """The 2*NDATA float array the sinogram reader returns (prompts, background)."""
+    nd = prompts.size
+    out = np.zeros(2 * nd, np.float32)
+    out[:nd] = prompts
+    if background is not None:
+        out[nd:] = background
+    return out

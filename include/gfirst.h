@@ -1,0 +1,256 @@
+/*
+ * gfirst.h — C ABI of the MI355X-native fully-3D OSEM hot path (libgfirst.so).
+ *
+ * The reference (GFIRST) has no library boundary: its path is a set of CUDA
+ * host wrappers that share global state (texture references, cudaArrays,
+ * __constant__ taps) and exit the process on error.  Each entry point below
+ * replaces one of those wrappers (reference file:line given per function);
+ * GFIRST_GPU_Kernels.cu is abbreviated "GPU:", GFIRST.cu "DRV:" and
+ * GFIRST_CPU_Kernels.cpp "CPU:".
+ *
+ * Conventions
+ *  - Every entry returns an int status: 0 ok, <0 error (GFR_E*); nothing exits.
+ *    The message of the last error on a context is gfr_last_error(ctx).
+ *  - Geometry travels by value (pointer to a caller-owned record) in every
+ *    call, so one context serves studies of different scanner geometries.
+ *  - Device buffers are caller-owned (gfr_alloc/gfr_free); calls taking a
+ *    `stream` argument are asynchronous and ordered on that HIP stream
+ *    (hipStream_t passed as void*, NULL = the context's own stream).
+ *  - A context may be used by one host thread at a time.
+ *  - Layouts: image [z][y][x] (x fastest, NVOX = res*res*nzs floats);
+ *    sinogram [s][theta][r] (r fastest, NDATA = nsinogs*nang*nrad), with
+ *    sinograms ordered by segment 0,-1,+1,-2,+2,... then plane (DRV:133-140).
+ *  - Subsets: subset s of S holds the angles theta with theta % S == s, all
+ *    segments; S = 1 is the reference's full MLEM pass.
+ */
+#ifndef GFIRST_H
+#define GFIRST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFR_OK 0
+#define GFR_EINVAL -1    /* bad argument or geometry                */
+#define GFR_EHIP -2      /* HIP runtime error                       */
+#define GFR_ENOMEM -3    /* allocation failed or over the HBM budget */
+#define GFR_EIO -4       /* file missing/unreadable                 */
+#define GFR_ESTATE -5    /* call not valid in the object's state     */
+
+/* Scanner / sinogram / image geometry. The reference compiles these in
+ * (GFIRST.h:4-16, 31-34); gfr_geometry_default() returns those values. */
+typedef struct gfr_geometry {
+  int res;        /* RES: image x/y size                                  */
+  int nzs;        /* NZS: image slices == direct sinogram planes          */
+  int nrad;       /* NRAD: radial bins                                    */
+  int nang;       /* NANG: angles over 180 degrees                        */
+  int span;       /* SPAN                                                 */
+  int nseg;       /* NSEGMENTS (odd: 0, -1, +1, ...)                      */
+  int ntbins;     /* NTBINS: must be 1 (no TOF on this path)              */
+  float pitch;    /* crystal pitch, mm                                    */
+  float diam_det; /* detector diameter, mm                                */
+  float fov;      /* transaxial field of view, mm                         */
+} gfr_geometry;
+
+typedef struct gfr_geometry_info {
+  int nsinogs;      /* NSINOGS derived from the segment table             */
+  long long ndata;  /* nsinogs*nang*nrad                                  */
+  long long nvox;   /* res*res*nzs                                        */
+  int lambda0;      /* first sample index along a LOR (GPU:90)            */
+  int lambda1;      /* last sample index (inclusive, GPU:91)              */
+} gfr_geometry_info;
+
+/* Storage precision of measured sinograms inside a reconstruction. */
+#define GFR_STORE_F32 0
+#define GFR_STORE_F16 1
+#define GFR_STORE_BF16 2
+
+/* Reconstruction parameters (GFIRST.h:20-30; DRV:512-517). */
+typedef struct gfr_recon_params {
+  int mode;          /* 1 doubles; 2 doubles+triples sinograms; 3 triples from VLORs */
+  int niter;         /* iterations (reference NITER = 60)                  */
+  int nsub;          /* subsets S (1 = reference)                          */
+  float psf_fw;      /* PSF_FW, voxels                                     */
+  float psf_bw;      /* PSF_BW, voxels                                     */
+  int jbf_r;         /* joint bilateral radius (4)                         */
+  float jbf_sigma3d; /* 0.75                                               */
+  float jbf_sigmai;  /* 0.025                                              */
+  float jbf_coef;    /* blend coefficient (0.1)                            */
+  int storage;       /* GFR_STORE_*: sinogram storage in HBM               */
+} gfr_recon_params;
+
+typedef struct gfr_ctx gfr_ctx;
+typedef struct gfr_vlor gfr_vlor;
+typedef struct gfr_recon gfr_recon;
+
+/* ---------------- geometry (host only; no device needed) ---------------- */
+
+/* Compiled constants of the reference (GFIRST.h:4-16, 31-34). */
+int gfr_geometry_default(gfr_geometry* g);
+/* Validates g and derives sizes (replaces the hand-set NSINOGS, GFIRST.h:9). */
+int gfr_geometry_check(const gfr_geometry* g, gfr_geometry_info* info);
+/* Segment table: zmin, sign and cumulative sinogram offsets (nseg+1) in the
+ * reference's segment order (DRV:133-140, 163-168; GPU:54). Returns nsinogs. */
+int gfr_seg_tables(const gfr_geometry* g, int* zmin, int* sgn, int* nz_acum);
+/* Per-angle sin/cos of theta = -it/NANG*PI + PI (GPU:75-77, 139-141), from
+ * libm in double rounded to float. */
+int gfr_angle_tables(const gfr_geometry* g, float* sin_t, float* cos_t);
+/* 9-tap PSF coefficients (GPU:322-330, calcGaussianCoefficients). */
+int gfr_psf_taps(float psf, float* g9);
+/* 5-tap z coefficients (CPU:169-173, gaussian_filterZ). */
+int gfr_gz_taps(float* g5);
+
+/* ---------------- context and memory ---------------- */
+
+/* Replaces the implicit CUDA context + crearTexturas() (GPU:254-281) and the
+ * cudaMalloc block of osem3d (DRV:62-83). hbm_budget = 0: no limit. */
+int gfr_ctx_create(int device, size_t hbm_budget, gfr_ctx** out);
+int gfr_ctx_destroy(gfr_ctx* ctx);
+/* Replaces chequearError (GPU:305-320): returns the message, never exits. */
+const char* gfr_last_error(gfr_ctx* ctx);
+void* gfr_ctx_stream(gfr_ctx* ctx);
+int gfr_sync(gfr_ctx* ctx, void* stream);
+int gfr_alloc(gfr_ctx* ctx, size_t bytes, void** dptr);
+int gfr_free(gfr_ctx* ctx, void* dptr);
+/* Bytes currently allocated through gfr_alloc / reconstructions. */
+size_t gfr_bytes_in_use(gfr_ctx* ctx);
+/* Replaces UpdateImage / actualizarCorr / cudaMemcpy (GPU:221-252). */
+int gfr_memcpy_h2d(gfr_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
+int gfr_memcpy_d2h(gfr_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
+/* Zero fill with a kernel (CleanMemory, GPU:207-219). */
+int gfr_zero(gfr_ctx* ctx, void* dptr, size_t bytes, void* stream);
+
+/* ---------------- per-operation entry points (stream ordered) ---------------- */
+
+/* CreateInitialCylinder (CPU:35-56) into a device image. */
+int gfr_fill_cylinder(gfr_ctx* ctx, const gfr_geometry* g, float val, float* d_img, void* stream);
+
+/* convolutionXY with calcGaussianCoefficients(psf) (GPU:322-364): per plane a
+ * 9-tap y filter (zero padded) then a 9-tap x filter for 4<=x<=RES-5; other
+ * columns keep the raw input. d_in != d_out. */
+int gfr_blur_xy(gfr_ctx* ctx, const gfr_geometry* g, float psf, const float* d_in, float* d_out,
+                void* stream);
+
+/* Forward projection of every view of subset `subset` (FORWPROJ + host
+ * scatter, GPU:40-107, DRV:346-363): writes the subset's bins of d_sino
+ * (NDATA layout); other bins are untouched. */
+int gfr_forward_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_img, float* d_sino,
+                        int subset, int nsub, void* stream);
+
+/* Calibration mask (DRV:453-458): mask = 1 where PROJ_BLANK>0.01 and
+ * norm>0.01, else 0. d_mask is NDATA bytes. */
+int gfr_calib_mask(gfr_ctx* ctx, const gfr_geometry* g, const float* d_proj_blank,
+                   const float* d_norm, uint8_t* d_mask, void* stream);
+
+/* Ratio (DRV:413-430) over the subset's bins: corr = y/(p+max(bg,0)) if
+ * p+bg >= 0.01 else -1000. d_bg may be NULL (no background). */
+int gfr_ratio(gfr_ctx* ctx, const gfr_geometry* g, const float* d_proj, const float* d_y,
+              const float* d_bg, float* d_corr, int subset, int nsub, void* stream);
+
+/* Fused forward projection + ratio: writes d_corr (and the projection into
+ * d_proj when non-NULL) for the subset's bins. Same results as
+ * gfr_forward_project followed by gfr_ratio. */
+int gfr_forward_ratio(gfr_ctx* ctx, const gfr_geometry* g, const float* d_img, const float* d_y,
+                      const float* d_bg, float* d_corr, float* d_proj, int subset, int nsub,
+                      void* stream);
+
+/* Backprojection of the subset's views (per-view loop of DRV:436-469 with
+ * BACKWPROJ, GPU:109-171): d_numer/d_denom receive the complete subset sums
+ * (the reference's CleanMemory + per-view accumulation), voxel by voxel in
+ * the reference's segment-major, angle-minor order. */
+int gfr_back_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_corr,
+                     const uint8_t* d_mask, float* d_numer, float* d_denom, int subset, int nsub,
+                     void* stream);
+
+/* Image update (DRV:482-502; GPU:173-188; CPU:165-187): R = DENOM<0.001 ? 1 :
+ * NUMER/DENOM; F = Gz(Gxy_psf_bw(R)); x *= F; slice 0 := slice 1 and slice
+ * NZS-1 := slice NZS-2. d_work is NVOX floats of scratch. In place on d_x. */
+int gfr_update(gfr_ctx* ctx, const gfr_geometry* g, float psf_bw, const float* d_numer,
+               const float* d_denom, float* d_x, float* d_work, void* stream);
+
+/* jointbilateralfilter3D (GPU:508-544). */
+int gfr_jbf3d(gfr_ctx* ctx, const gfr_geometry* g, const float* d_in, const float* d_guide,
+              float max_ref, int r, float sigma3d, float sigmai, float* d_out, void* stream);
+/* Triples blend x = (1-coef)*x + coef*filtered (DRV:524). */
+int gfr_blend(gfr_ctx* ctx, const gfr_geometry* g, float coef, const float* d_filt, float* d_x,
+              void* stream);
+/* max(0, max_i x[i]) (DRV:521); synchronous. */
+int gfr_image_max(gfr_ctx* ctx, const gfr_geometry* g, const float* d_x, float* h_max,
+                  void* stream);
+
+/* VLOR soft assignment (DRV:229-247 pre-count and BLANKT; DRV:373-388 per
+ * pass). `rec` is the record sequence the reference's read loop processes
+ * (pairs of 1-based bin indices, last record already repeated; see
+ * gfr_read_vlor_file). d_yT = triples prompts already scaled by RES. */
+int gfr_vlor_create(gfr_ctx* ctx, const gfr_geometry* g, const int32_t* rec, long long nrec,
+                    const float* d_yT, gfr_vlor** out);
+/* d_dv[b] for the subset's bins b: sum over records touching b, in record
+ * order, of proj[b]/(proj[i1]+proj[i2])*(BLANKT[i1]+BLANKT[i2]). */
+int gfr_vlor_assign(gfr_ctx* ctx, gfr_vlor* v, const float* d_proj, float* d_dv, int subset,
+                    int nsub, void* stream);
+int gfr_vlor_destroy(gfr_vlor* v);
+
+/* ---------------- raw files (host; CPU:104-142, DRV:229-240) ---------------- */
+
+/* ReadSinogram_Type: out = 2*ndata floats (prompts, background); int32 file
+ * unless the int view's max exceeds 1e6, then float32. A missing file gives
+ * zeros and returns 1 (the reference prints "Sinogram Not Found!"); words
+ * beyond 2*ndata are ignored and reported by *truncated. Returns 0/1 or <0. */
+int gfr_read_sinogram_file(const char* path, long long ndata, float* out, int* is_float,
+                           int* truncated);
+/* VLOR file -> record sequence (last record repeated as by while(!feof)).
+ * Call with rec=NULL to get the count; returns count or <0. */
+long long gfr_read_vlor_file(const char* path, int32_t* rec, long long cap);
+
+/* ---------------- whole reconstruction (device-resident session) ---------------- */
+
+/* osem3d (DRV:44-575) as a session: uploads the data once, keeps every
+ * array resident in HBM, and advances one (type, subset) pass per step.
+ * d2/t2: 2*ndata floats as read from file (unscaled); t2 NULL in mode 1;
+ * norm: ndata floats; rec/nrec: VLOR records (mode 3). Admission checks the
+ * context's HBM budget. */
+int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params* p,
+                     const float* d2, const float* t2, const float* norm, const int32_t* rec,
+                     long long nrec, gfr_recon** out);
+/* Device bytes a reconstruction with these parameters holds. */
+long long gfr_recon_footprint(const gfr_geometry* g, const gfr_recon_params* p);
+/* One (type, subset) pass; returns 1 when the reconstruction is complete. */
+int gfr_recon_step(gfr_recon* r);
+/* Run the remaining passes. */
+int gfr_recon_run(gfr_recon* r);
+/* Position: iteration (1-based), type (0 D, 1 T), subset; total passes done. */
+int gfr_recon_cursor(gfr_recon* r, int* iter, int* type, int* subset, long long* passes);
+/* Copy state to the host: what = 0 doubles image, 1 triples image,
+ * 2 PROJ_BLANK (NDATA), 3 VLOR pseudo-sinogram before scaling (NDATA). */
+#define GFR_OUT_XD 0
+#define GFR_OUT_XT 1
+#define GFR_OUT_PROJ_BLANK 2
+#define GFR_OUT_VLOR 3
+int gfr_recon_get(gfr_recon* r, int what, float* host_out);
+/* FP64 Poisson log-likelihood sum_b (y ln p - p) over the last pass's bins. */
+int gfr_recon_loglik(gfr_recon* r, double* out);
+void* gfr_recon_stream(gfr_recon* r);
+int gfr_recon_destroy(gfr_recon* r);
+
+/* Device-time tracing per phase of a pass (HIP events on the session's
+ * stream; the reference times only with time() at 1 s resolution,
+ * DRV:51-52, 572-573). Each phase is one kernel launch except UPDATE (two). */
+#define GFR_PHASE_BLUR 0    /* PSF-FW blur                               */
+#define GFR_PHASE_FORWARD 1 /* forward projection (+ ratio, FP64 LLH)    */
+#define GFR_PHASE_VLOR 2    /* VLOR reassignment + ratio (mode 3)        */
+#define GFR_PHASE_BACK 3    /* backprojection                            */
+#define GFR_PHASE_UPDATE 4  /* ratio blur, z filter, multiply            */
+#define GFR_PHASE_JBF 5     /* max + joint bilateral filter + blend       */
+#define GFR_NPHASES 6
+int gfr_recon_profile(gfr_recon* r, int enable);
+/* Waits for the session's stream, then returns the accumulated milliseconds
+ * and pass counts per phase since the last call (arrays of GFR_NPHASES). */
+int gfr_recon_phase_times(gfr_recon* r, double* ms, long long* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,250 @@
+"""GPU parity: every HIP kernel and the whole reconstruction against the CPU
+oracle on the same seeded inputs, through the C ABI.
+
+Tolerances: the kernels and the oracle share one rounding model (explicit
+fmaf, IEEE division, host-tabled sin/cos), so single operators are compared
+at max relative error 1e-6 (and reported when bit-exact); whole iterations at
+relative L2 1e-4 in FP32 (the bound the path is specified to).
+"""
+import numpy as np
+import pytest
+
+import gfirst_amd as ga
+from gfirst_amd.geometry import C0, C1, SMALL, TINY
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (d if d > 0 else 1.0)
+
+
+def close(got, exp, rtol=1e-6, atol=0.0, what=""):
+    got = np.asarray(got)
+    exp = np.asarray(exp)
+    assert got.shape == exp.shape
+    bad = ~np.isclose(got, exp, rtol=rtol, atol=atol)
+    nexact = int(np.sum(got == exp))
+    print("%s: %d/%d bit-exact, max|d|=%.3g" % (what, nexact, got.size,
+                                                float(np.max(np.abs(got.astype(np.float64) - exp)))
+                                                if got.size else 0.0))
+    assert not bad.any(), "%s: %d mismatches, first at %d: %r vs %r" % (
+        what, bad.sum(), np.argmax(bad), got.ravel()[np.argmax(bad)], exp.ravel()[np.argmax(bad)])
+
+
+def rand_img(g, seed, lo=0.0, hi=2.0):
+    return np.random.default_rng(seed).uniform(lo, hi, g.nvox).astype(np.float32)
+
+
+@pytest.mark.parametrize("g", [TINY, C0], ids=["tiny", "c0"])
+def test_blur_xy(gctx, orc, g):
+    x = rand_img(g, 1)
+    for psf in (1.5, 0.7):
+        close(gctx.blur_xy(g, x, psf), orc.blur_xy(g, x, psf), what="blur")
+
+
+@pytest.mark.parametrize("g", [TINY, SMALL, C0], ids=["tiny", "small", "c0"])
+def test_forward_project(gctx, orc, g):
+    x = rand_img(g, 2)
+    close(gctx.forward_project(g, x), orc.forward(g, x), what="fwd S=1")
+    close(gctx.forward_project(g, x, 1, 4), orc.forward(g, x, 1, 4), what="fwd s=1/4")
+
+
+def test_forward_constant_image_c1(gctx):
+    """Size-independent property at Inveon scale: RES+1 on every bin."""
+    g = C1
+    x = gctx.to_device(np.ones(g.nvox, np.float32))
+    out = gctx.array(g.ndata).zero()
+    gctx.forward_project(g, x, 3, 16, out=out)
+    s = out.download().reshape(-1, g.nang, g.nrad)
+    assert np.all(s[:, 3::16] == np.float32(g.res + 1))
+    mask = np.ones(g.nang, bool)
+    mask[3::16] = False
+    assert np.all(s[:, mask] == 0)
+
+
+def test_forward_c1_single_angle_vs_oracle(gctx, orc):
+    g = C1
+    x = rand_img(g, 3)
+    close(gctx.forward_project(g, x, 37, 160), orc.forward(g, x, 37, 160), what="fwd c1 1 angle")
+
+
+def test_calib_mask_and_ratio(gctx, orc):
+    g = C0
+    rng = np.random.default_rng(4)
+    nd = g.ndata
+    pb = rng.uniform(0, 0.03, nd).astype(np.float32)
+    nm = rng.uniform(0, 0.03, nd).astype(np.float32)
+    m = gctx.calib_mask(g, pb, nm)
+    cal = orc.calib(g, pb, nm)
+    assert np.array_equal(m.astype(bool), cal > 0)
+    proj = rng.uniform(-0.02, 5, nd).astype(np.float32)
+    y = (rng.poisson(2.0, nd) * g.res).astype(np.float32)
+    bg = rng.uniform(-1, 1, nd).astype(np.float32)
+    for s, S in ((0, 1), (2, 4)):
+        got = gctx.ratio(g, proj, y, bg, s, S)
+        exp = orc.ratio(g, proj, y, bg, s, S)
+        close(got, exp, rtol=0, what="ratio")
+
+
+def test_forward_ratio_fused(gctx, orc):
+    g = C0
+    x = rand_img(g, 5)
+    rng = np.random.default_rng(6)
+    nd = g.ndata
+    y = (rng.poisson(3.0, nd) * g.res).astype(np.float32)
+    bg = rng.uniform(0, 2, nd).astype(np.float32)
+    got = gctx.forward_ratio(g, x, y, bg, 1, 4)
+    exp = orc.ratio(g, orc.forward(g, x, 1, 4), y, bg, 1, 4)
+    close(got, exp, what="forward+ratio")
+
+
+@pytest.mark.parametrize("g", [SMALL, C0], ids=["small", "c0"])
+def test_back_project(gctx, orc, g):
+    rng = np.random.default_rng(7)
+    nd = g.ndata
+    corr = rng.uniform(0, 3, nd).astype(np.float32)
+    corr[rng.uniform(size=nd) < 0.05] = -1000.0
+    valid = rng.uniform(size=nd) < 0.93
+    cal = np.where(valid, 1, -1000).astype(np.float32)
+    for s, S in ((0, 1), (3, 4)):
+        n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
+        ne, de = orc.back(g, corr, cal, s, S)
+        close(n, ne, rtol=1e-6, atol=1e-6, what="numer")
+        close(d, de, rtol=1e-6, atol=1e-6, what="denom")
+
+
+def test_back_project_c1_subset_vs_oracle(gctx, orc):
+    g = C1
+    rng = np.random.default_rng(8)
+    nd = g.ndata
+    corr = rng.uniform(0, 3, nd).astype(np.float32)
+    valid = rng.uniform(size=nd) < 0.97
+    cal = np.where(valid, 1, -1000).astype(np.float32)
+    n, d = gctx.back_project(g, corr, valid.astype(np.uint8), 5, 80)
+    ne, de = orc.back(g, corr, cal, 5, 80)
+    close(n, ne, rtol=1e-6, atol=1e-5, what="numer c1")
+    close(d, de, rtol=1e-6, atol=1e-5, what="denom c1")
+
+
+@pytest.mark.parametrize("g", [SMALL, C0], ids=["small", "c0"])
+def test_update(gctx, orc, g):
+    rng = np.random.default_rng(9)
+    num = rng.uniform(0, 50, g.nvox).astype(np.float32)
+    den = rng.uniform(0, 50, g.nvox).astype(np.float32)
+    den[::13] = 0.0
+    x = rng.uniform(0.1, 2, g.nvox).astype(np.float32)
+    got = gctx.update(g, x, num, den, 1.5)
+    exp = orc.update(g, x, num, den, 1.5)
+    close(got, exp, what="update")
+
+
+def test_jbf_blend_max(gctx, orc):
+    g = SMALL
+    rng = np.random.default_rng(10)
+    inp = rng.uniform(0, 2, g.nvox).astype(np.float32)
+    guide = rng.uniform(0, 1, g.nvox).astype(np.float32)
+    mx = gctx.image_max(g, guide)
+    assert mx == orc.vmax(guide)
+    got = gctx.jbf3d(g, inp, guide, mx)
+    exp = orc.jbf(g, inp, guide, mx)
+    close(got, exp, rtol=2e-6, atol=1e-7, what="jbf")
+    b = gctx.blend(g, inp, exp, 0.1)
+    close(b, orc.blend(g, inp, exp, 0.1), rtol=0, what="blend")
+
+
+def test_vlor_assign(gctx, orc):
+    g = SMALL
+    nd = g.ndata
+    rng = np.random.default_rng(11)
+    rec = rng.integers(-3, nd + 3, size=(4000, 2)).astype(np.int32)
+    rec[:50, 1] = rec[:50, 0]  # same bin twice
+    yT = (rng.poisson(1.0, nd) * g.res).astype(np.float32)
+    proj = rng.uniform(0, 2, nd).astype(np.float32)
+    proj[::5] = 0.0
+    v = gctx.vlor(g, rec, yT)
+    bt = orc.vlor_blankt(g, yT, orc.vlor_precount(g, rec))
+    for s, S in ((0, 1), (1, 3)):
+        got = v.assign(proj, s, S)
+        exp = orc.vlor_assign(g, rec, bt, proj, s, S)
+        close(got, exp, what="vlor")
+    v.close()
+
+
+def make_study(g, seed, mode, counts=2e5):
+    from gfirst_amd import synth
+    rng = np.random.default_rng(seed)
+    nd = g.ndata
+    lam = rng.uniform(0.2, 3.0, nd) * (counts / nd)
+    d2 = synth.as_file_pair(rng.poisson(lam).astype(np.float32),
+                            rng.uniform(0, 0.3, nd).astype(np.float32))
+    t2 = None
+    if mode >= 2:
+        t2 = synth.as_file_pair(rng.poisson(lam * 0.1).astype(np.float32))
+    norm = rng.uniform(0.8, 1.2, nd).astype(np.float32)
+    norm[rng.uniform(size=nd) < 0.02] = 0.0
+    rec = None
+    if mode == 3:
+        rec = rng.integers(1, nd, size=(3000, 2)).astype(np.int32)
+        rec = np.concatenate([rec, rec[-1:]])
+    return d2, t2, norm, rec
+
+
+@pytest.mark.parametrize("g,mode,S,niter", [(C0, 1, 4, 2), (SMALL, 2, 3, 2), (SMALL, 3, 1, 2),
+                                             (SMALL, 3, 4, 1)],
+                         ids=["c0-m1-S4", "small-m2-S3", "small-m3-S1", "small-m3-S4"])
+def test_recon_vs_oracle(gctx, orc, g, mode, S, niter):
+    d2, t2, norm, rec = make_study(g, 12 + mode, mode)
+    p = ga.recon_params(mode=mode, niter=niter, nsub=S)
+    r = gctx.recon(g, p, d2, t2, norm, rec)
+    pb = r.get(ga.OUT_PROJ_BLANK)
+    r.run()
+    xD = r.get(ga.OUT_XD)
+    op = orc.params(mode=mode, niter=niter, nsub=S)
+    exD, exT, epb, _ = orc.osem(g, op, d2, t2, norm, rec)
+    close(pb, epb, what="proj_blank")
+    e = rel_l2(xD, exD)
+    print("xD rel L2 vs oracle FP32: %.3g (bit-exact %d/%d)" % (e, np.sum(xD == exD), xD.size))
+    assert e <= 1e-4
+    if mode >= 2:
+        xT = r.get(ga.OUT_XT)
+        eT = rel_l2(xT, exT)
+        print("xT rel L2 vs oracle FP32: %.3g" % eT)
+        assert eT <= 1e-4
+    r.close()
+
+
+def test_recon_loglik_and_cursor(gctx):
+    g = TINY
+    d2, _, norm, _ = make_study(g, 20, 1, counts=5e4)
+    r = gctx.recon(g, ga.recon_params(mode=1, niter=2, nsub=2), d2, None, norm)
+    assert r.cursor()["passes"] == 0
+    assert r.step() == 0
+    ll1 = r.loglik()
+    r.step()
+    r.step()
+    ll3 = r.loglik()
+    c = r.cursor()
+    assert c["passes"] == 3 and c["iter"] == 2 and c["subset"] == 1
+    assert np.isfinite(ll1) and np.isfinite(ll3)
+    assert r.step() == 1 and r.cursor()["done"]
+    r.close()
+
+
+def test_errors_are_status_codes(gctx):
+    g = TINY
+    x = gctx.array(g.nvox).zero()
+    with pytest.raises(ga.GfrError) as e:
+        gctx.forward_project(g, x, 3, 2)
+    assert e.value.code == ga.GFR_EINVAL
+    with pytest.raises((ga.GfrError, ValueError)):
+        gctx.forward_project(g.with_(ntbins=3), x)
+    small = ga.Context(0, hbm_budget=1 << 20)
+    d2, _, norm, _ = make_study(C0, 1, 1)
+    with pytest.raises(ga.GfrError) as e:
+        small.recon(C0, ga.recon_params(), d2, None, norm)
+    assert e.value.code == ga.GFR_ENOMEM
+    small.close()
