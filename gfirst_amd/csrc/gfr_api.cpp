@@ -25,6 +25,48 @@ int hip_err(gfr_ctx* ctx, hipError_t e, const char* where) {
 
 hipStream_t stream_of(gfr_ctx* ctx, void* s) { return s ? (hipStream_t)s : ctx->stream; }
 
+void* scratch_acquire(gfr_ctx* ctx, Scratch& s, size_t bytes, hipStream_t st, hipError_t* err) {
+  (void)ctx;
+  *err = hipSuccess;
+  if (!s.ev) {
+    *err = hipEventCreateWithFlags(&s.ev, hipEventDisableTiming);
+    if (*err != hipSuccess) return nullptr;
+  }
+  if (s.cap < bytes) {
+    if (s.p) {
+      (void)hipEventSynchronize(s.ev);
+      (void)hipFree(s.p);
+      s.p = nullptr;
+      s.cap = 0;
+      s.used = false;
+    }
+    *err = hipMalloc(&s.p, bytes);
+    if (*err != hipSuccess) {
+      s.p = nullptr;
+      return nullptr;
+    }
+    s.cap = bytes;
+  }
+  if (s.used) {
+    *err = hipStreamWaitEvent(st, s.ev, 0);
+    if (*err != hipSuccess) return nullptr;
+  }
+  return s.p;
+}
+
+void scratch_release(Scratch& s, hipStream_t st) {
+  if (hipEventRecord(s.ev, st) == hipSuccess) s.used = true;
+}
+
+void scratch_free(Scratch& s) {
+  if (s.ev) {
+    (void)hipEventSynchronize(s.ev);
+    (void)hipEventDestroy(s.ev);
+  }
+  if (s.p) (void)hipFree(s.p);
+  s = Scratch();
+}
+
 GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g) {
   if (!g) {
     set_err(ctx, GFR_EINVAL, "geometry is NULL");
@@ -46,12 +88,14 @@ GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g) {
   const size_t off_cos = n_ang * sizeof(float);
   const size_t off_seg = ((2 * n_ang * sizeof(float)) + 15) & ~size_t(15);
   const size_t off_sino = (off_seg + n_seg * sizeof(SegEntry) + 15) & ~size_t(15);
-  const size_t total = off_sino + n_sino * sizeof(int2);
+  const size_t off_zt = (off_sino + n_sino * sizeof(int2) + 15) & ~size_t(15);
+  const size_t total = off_zt + h.ztab.size() * sizeof(uint2);
   std::vector<char> blob(total, 0);
   memcpy(blob.data(), h.sin_t.data(), n_ang * sizeof(float));
   memcpy(blob.data() + off_cos, h.cos_t.data(), n_ang * sizeof(float));
   memcpy(blob.data() + off_seg, h.seg.data(), n_seg * sizeof(SegEntry));
   memcpy(blob.data() + off_sino, h.sino.data(), n_sino * sizeof(int2));
+  memcpy(blob.data() + off_zt, h.ztab.data(), h.ztab.size() * sizeof(uint2));
   hipError_t e = hipMalloc(&gd->block, total);
   if (e == hipSuccess) e = hipMemcpy(gd->block, blob.data(), total, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -80,6 +124,7 @@ GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g) {
   d.cos_t = (const float*)(base + off_cos);
   d.seg = (const SegEntry*)(base + off_seg);
   d.sino = (const int2*)(base + off_sino);
+  d.ztab = (const uint2*)(base + off_zt);
   ctx->geoms[key] = gd;
   return gd;
 }
@@ -96,6 +141,24 @@ bool bad_subset(gfr_ctx* ctx, const GeomDev* gd, int subset, int nsub) {
     return true;
   }
   return false;
+}
+
+// Per-operation forward projection through the context's rotation scratch.
+int run_forward(gfr_ctx* ctx, GeomDev* gd, const float* d_img, const FwdOut& o, int subset,
+                int nsub, hipStream_t st) {
+  const int nav = (gd->dev.nang - subset + nsub - 1) / nsub;
+  const size_t per = rotate_floats_per_angle(gd->dev);
+  const size_t floats = per * (size_t)std::min(nav, kRotChunk);
+  hipError_t e = hipSuccess;
+  float* R;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    R = (float*)scratch_acquire(ctx, ctx->rot, floats * sizeof(float), st, &e);
+    if (R) e = launch_forward(gd->dev, d_img, R, floats, o, subset, nsub, st);
+    if (R) scratch_release(ctx->rot, st);
+  }
+  if (e != hipSuccess) return hip_err(ctx, e, "forward projection");
+  return GFR_OK;
 }
 
 #define GFR_CHECK_CTX(ctx) \
@@ -143,6 +206,8 @@ int gfr_ctx_destroy(gfr_ctx* ctx) {
   }
   for (auto& kv : ctx->allocs) (void)hipFree(kv.first);
   (void)hipFree(ctx->d_scratch_u32);
+  scratch_free(ctx->rot);
+  scratch_free(ctx->pack);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return GFR_OK;
@@ -246,9 +311,9 @@ int gfr_forward_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_img,
   GFR_NONNULL(ctx, d_img, "d_img");
   GFR_NONNULL(ctx, d_sino, "d_sino");
   if (bad_subset(ctx, gd, subset, nsub)) return GFR_EINVAL;
-  GFR_HIP(ctx, launch_forward(gd->dev, d_img, d_sino, nullptr, nullptr, GFR_STORE_F32, 1.f,
-                              nullptr, nullptr, subset, nsub, stream_of(ctx, stream)));
-  return GFR_OK;
+  FwdOut o;
+  o.proj = d_sino;
+  return run_forward(ctx, gd, d_img, o, subset, nsub, stream_of(ctx, stream));
 }
 
 int gfr_calib_mask(gfr_ctx* ctx, const gfr_geometry* g, const float* d_proj_blank,
@@ -270,7 +335,7 @@ int gfr_ratio(gfr_ctx* ctx, const gfr_geometry* g, const float* d_proj, const fl
   GFR_NONNULL(ctx, d_y, "d_y");
   GFR_NONNULL(ctx, d_corr, "d_corr");
   if (bad_subset(ctx, gd, subset, nsub)) return GFR_EINVAL;
-  GFR_HIP(ctx, launch_ratio(gd->dev, d_proj, d_y, d_bg, GFR_STORE_F32, 1.f, d_corr, subset, nsub,
+  GFR_HIP(ctx, launch_ratio(gd->dev, d_proj, d_y, d_bg, GFR_STORE_F32, 1.f, d_corr, nullptr, nullptr, subset, nsub,
                             stream_of(ctx, stream)));
   return GFR_OK;
 }
@@ -284,9 +349,12 @@ int gfr_forward_ratio(gfr_ctx* ctx, const gfr_geometry* g, const float* d_img, c
   GFR_NONNULL(ctx, d_y, "d_y");
   GFR_NONNULL(ctx, d_corr, "d_corr");
   if (bad_subset(ctx, gd, subset, nsub)) return GFR_EINVAL;
-  GFR_HIP(ctx, launch_forward(gd->dev, d_img, d_proj, d_y, d_bg, GFR_STORE_F32, 1.f, d_corr,
-                              nullptr, subset, nsub, stream_of(ctx, stream)));
-  return GFR_OK;
+  FwdOut o;
+  o.proj = d_proj;
+  o.corr = d_corr;
+  o.y = d_y;
+  o.bg = d_bg;
+  return run_forward(ctx, gd, d_img, o, subset, nsub, stream_of(ctx, stream));
 }
 
 int gfr_back_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_corr,
@@ -299,8 +367,19 @@ int gfr_back_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_corr,
   GFR_NONNULL(ctx, d_numer, "d_numer");
   GFR_NONNULL(ctx, d_denom, "d_denom");
   if (bad_subset(ctx, gd, subset, nsub)) return GFR_EINVAL;
-  GFR_HIP(ctx, launch_back(gd->dev, d_corr, d_mask, d_numer, d_denom, subset, nsub,
-                           stream_of(ctx, stream)));
+  // Pack the subset's {ratio, calib} rows, then the session's backprojector.
+  hipStream_t st = stream_of(ctx, stream);
+  const size_t bytes = packed_bytes(gd->dev, subset, nsub);
+  hipError_t e = hipSuccess;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    float2* c2 = (float2*)scratch_acquire(ctx, ctx->pack, bytes, st, &e);
+    if (c2) e = launch_pack(gd->dev, d_corr, d_mask, c2, subset, nsub, st);
+    if (c2 && e == hipSuccess)
+      e = launch_back_packed(gd->dev, c2, d_numer, d_denom, subset, nsub, st);
+    if (c2) scratch_release(ctx->pack, st);
+  }
+  if (e != hipSuccess) return hip_err(ctx, e, "back projection");
   return GFR_OK;
 }
 

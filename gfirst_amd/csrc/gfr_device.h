@@ -1,0 +1,78 @@
+// gfr_device.h — device helpers shared by the gfx950 kernels.
+//
+// Rounding model (shared with the CPU oracle so results compare bit for bit):
+// every kernel file is compiled with -ffp-contract=off; the only fused
+// multiply-adds are explicit fmaf() calls. Interpolation weights are exact
+// FP32: lerp(a,b,t) = fma(t, b-a, a), x first, then y, then z.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gfr_internal.h"
+
+namespace gfr {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ float lerp1(float a, float b, float t) { return fmaf(t, b - a, a); }
+
+// Unnormalized coordinate -> (i0, i1, alpha) with clamp-to-edge addressing:
+// texel i covers [i, i+1), so the filter blends floor(c-0.5) and the next one
+// (the reference's textures use linear filtering + clamp,
+// GFIRST_GPU_Kernels.cu:260-263, 270-279).
+__device__ __forceinline__ void axis(float c, int n, int& i0, int& i1, float& a) {
+  const float t = c - 0.5f;
+  const float f = floorf(t);
+  const int i = (int)f;
+  a = t - f;
+  i0 = min(max(i, 0), n - 1);
+  i1 = min(max(i + 1, 0), n - 1);
+}
+
+template <int S>
+__device__ __forceinline__ float load_store(const void* p, size_t i) {
+  if constexpr (S == GFR_STORE_F32) {
+    return ((const float*)p)[i];
+  } else if constexpr (S == GFR_STORE_F16) {
+    return (float)((const _Float16*)p)[i];
+  } else {
+    const uint32_t b = (uint32_t)((const uint16_t*)p)[i] << 16;
+    return __uint_as_float(b);
+  }
+}
+
+// Work-groups of a 1-D grid are dispatched round-robin over the 8 XCDs (each
+// with its own L2). This maps the hardware block id to a logical id so that
+// every XCD walks one contiguous range of logical blocks (bijective for any
+// grid size).
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
+  const unsigned nx = n >> 3, rem = n & 7u, x = b & 7u, k = b >> 3;
+  return x < rem ? x * (nx + 1) + k : rem * (nx + 1) + (x - rem) * nx + k;
+}
+
+// Packed {ratio, calib} rows of a subset for the backprojector. Per angle of
+// the subset and per segment, the segment's planes zmin..NZS-zmin-1 are
+// preceded and followed by one extra row, so that every plane a voxel can
+// blend (zmin-1 .. NZS-zmin) is addressable without a test:
+//  - segments with zmin > 0: the extra rows hold -1000, the value the
+//    reference stores outside the segment's planes (GFIRST.cu:447);
+//  - the direct segment (zmin = 0): the extra rows repeat planes 0 and NZS-1,
+//    which is what the clamp-to-edge filter reads for planes -1 and NZS.
+__device__ __forceinline__ size_t packed_rows_per_angle(const DevGeom& G) {
+  return (size_t)G.nsinogs + 2 * (size_t)G.nseg;
+}
+__device__ __forceinline__ void store_packed(const DevGeom& G, float2* c2, int ag, int s, int r,
+                                             float2 v) {
+  const int2 sp = G.sino[s];
+  const SegEntry se = G.seg[sp.x];
+  const size_t row0 = (size_t)ag * packed_rows_per_angle(G) + se.nza + 2 * (size_t)sp.x;
+  const int k = sp.y - se.zmin;
+  const int nz = G.nzs - 2 * se.zmin;
+  const float2 edge = se.zmin == 0 ? v : make_float2(-1000.f, -1000.f);
+  c2[(row0 + 1 + k) * G.nrad + r] = v;
+  if (k == 0) c2[row0 * G.nrad + r] = edge;
+  if (k == nz - 1) c2[(row0 + nz + 1) * G.nrad + r] = edge;
+}
+
+}  // namespace gfr

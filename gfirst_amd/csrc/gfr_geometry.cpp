@@ -83,6 +83,28 @@ int derive_geometry(const gfr_geometry* g, GeomHost* out, std::string* err) {
   for (int k = 0; k < g->nseg; k++)
     for (int p = h.zmin[k]; p < g->nzs - h.zmin[k]; p++)
       h.sino[h.nza[k] + p - h.zmin[k]] = make_int2(k, p);
+  // z interpolation of every LOR sample: z = zc + l*dz with zc = p + 0.5
+  // (GFIRST_GPU_Kernels.cu:88, 96), through the clamp-to-edge linear filter.
+  // Same float operations as the device code (fmaf is exactly rounded on both).
+  if (g->nzs > 0xFFFF) return fail("geometry: nzs too large");
+  const int nl = h.info.lambda1 - h.info.lambda0 + 1;
+  h.ztab.resize((size_t)acc * nl);
+  for (int s = 0; s < acc; s++) {
+    const float dz = h.seg[h.sino[s].x].dzf;
+    const float zc = (float)h.sino[s].y + 0.5f;
+    for (int li = 0; li < nl; li++) {
+      const float z = fmaf((float)(h.info.lambda0 + li), dz, zc);
+      const float t = z - 0.5f;
+      const float f = floorf(t);
+      const float a = t - f;
+      const int i = (int)f;
+      const int z0 = i < 0 ? 0 : (i > g->nzs - 1 ? g->nzs - 1 : i);
+      const int z1 = i + 1 < 0 ? 0 : (i + 1 > g->nzs - 1 ? g->nzs - 1 : i + 1);
+      uint32_t ab;
+      memcpy(&ab, &a, 4);
+      h.ztab[(size_t)s * nl + li] = make_uint2((uint32_t)z0 | ((uint32_t)z1 << 16), ab);
+    }
+  }
   return GFR_OK;
 }
 

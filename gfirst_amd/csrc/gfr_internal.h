@@ -35,6 +35,9 @@ struct DevGeom {
   const float* cos_t;     // [nang]
   const SegEntry* seg;    // [nseg]
   const int2* sino;       // [nsinogs]: (segment index, absolute plane)
+  // [nsinogs][l1-l0+1]: z interpolation of each LOR sample, x = z0 | z1 << 16,
+  // y = bits of alpha_z (GFIRST_GPU_Kernels.cu:88, 96 through the clamp filter)
+  const uint2* ztab;
 };
 
 struct Taps9 { float g[9]; };
@@ -48,6 +51,7 @@ struct GeomHost {
   std::vector<float> sin_t, cos_t;
   std::vector<SegEntry> seg;
   std::vector<int2> sino;
+  std::vector<uint2> ztab;
   float RADM, RESM, fxf, fxb;
 };
 
@@ -63,6 +67,17 @@ struct GeomDev {
 
 }  // namespace gfr
 
+namespace gfr {
+// Device scratch shared by calls on different streams: a user waits for the
+// previous user's event, so calls that share it are ordered.
+struct Scratch {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool used = false;
+};
+}  // namespace gfr
+
 struct gfr_ctx {
   int device = 0;
   size_t budget = 0;
@@ -72,8 +87,19 @@ struct gfr_ctx {
   std::map<std::string, gfr::GeomDev*> geoms;
   std::map<void*, size_t> allocs;
   unsigned int* d_scratch_u32 = nullptr;  // 256 words for small reductions
+  gfr::Scratch rot;                       // rotation stage of per-operation forward calls
+  gfr::Scratch pack;                      // packed ratio of per-operation backprojections
   std::mutex mu;
 };
+
+namespace gfr {
+// Returns scratch of at least `bytes`, ordered after its previous user.
+void* scratch_acquire(gfr_ctx* ctx, Scratch& s, size_t bytes, hipStream_t st, hipError_t* err);
+void scratch_release(Scratch& s, hipStream_t st);
+void scratch_free(Scratch& s);
+// Angles per rotation chunk.
+constexpr int kRotChunk = 16;
+}  // namespace gfr
 
 namespace gfr {
 
@@ -95,16 +121,35 @@ hipError_t launch_fill_cylinder(const DevGeom& G, float val, float* img, hipStre
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);
 hipError_t launch_blur_xy(const DevGeom& G, const Taps9& t, const float* in, float* out,
                           hipStream_t st);
-hipError_t launch_forward(const DevGeom& G, const float* img, float* proj, const void* y,
-                          const void* bg, int storage, float yscale, float* corr,
-                          double* llh_part, int subset, int nsub, hipStream_t st);
+// Forward projection outputs (any subset of them).
+struct FwdOut {
+  float* proj = nullptr;          // projection, full sinogram layout
+  float* corr = nullptr;          // ratio, full layout
+  float2* corr2 = nullptr;        // {ratio, calib} packed [angle of subset][s][r]
+  const uint8_t* mask = nullptr;  // calibration mask (needed with corr2)
+  const void* y = nullptr;        // measured prompts (storage format, unscaled)
+  const void* bg = nullptr;       // background or NULL
+  int storage = GFR_STORE_F32;
+  float yscale = 1.f;             // RES
+  double* llh_part = nullptr;     // forward_llh_blocks() doubles, or NULL
+};
+// Scratch of the rotation stage: floats per angle; the launcher processes the
+// subset's angles in chunks of r_cap_floats / rotate_floats_per_angle().
+size_t rotate_floats_per_angle(const DevGeom& G);
+int forward_llh_blocks(const DevGeom& G, int subset, int nsub);
+hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r_cap_floats,
+                          const FwdOut& out, int subset, int nsub, hipStream_t st);
+hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* numer, float* denom,
+                              int subset, int nsub, hipStream_t st);
 hipError_t launch_ratio(const DevGeom& G, const float* proj, const float* y, const void* bg,
-                        int storage, float yscale, float* corr, int subset, int nsub,
-                        hipStream_t st);
+                        int storage, float yscale, float* corr, float2* corr2,
+                        const uint8_t* mask, int subset, int nsub, hipStream_t st);
 hipError_t launch_calib_mask(const DevGeom& G, const float* pb, const float* norm,
                              uint8_t* mask, hipStream_t st);
-hipError_t launch_back(const DevGeom& G, const float* corr, const uint8_t* mask, float* numer,
-                       float* denom, int subset, int nsub, hipStream_t st);
+// Full-layout ratio + mask -> packed {ratio, calib} of the subset (packed_bytes).
+size_t packed_bytes(const DevGeom& G, int subset, int nsub);
+hipError_t launch_pack(const DevGeom& G, const float* corr, const uint8_t* mask, float2* c2,
+                       int subset, int nsub, hipStream_t st);
 hipError_t launch_update(const DevGeom& G, const Taps9& t9, const Taps5& t5, const float* numer,
                          const float* denom, float* x, float* work, hipStream_t st);
 hipError_t launch_jbf(const DevGeom& G, const float* in, const float* guide,

@@ -4,129 +4,21 @@
 // the file is compiled with -ffp-contract=off; the only fused multiply-adds
 // are explicit fmaf() calls. Interpolation weights are exact FP32:
 // lerp(a,b,t) = fma(t, b-a, a), x first, then y, then z.
-#include <hip/hip_runtime.h>
 #include <math.h>
 
-#include "gfr_internal.h"
+#include "gfr_device.h"
 
 namespace gfr {
 
 namespace {
-
-constexpr int kWave = 64;
-
-__device__ __forceinline__ float lerp1(float a, float b, float t) { return fmaf(t, b - a, a); }
-
-// Unnormalized coordinate -> (i0, i1, alpha) with clamp-to-edge addressing:
-// texel i covers [i, i+1), so the filter blends floor(c-0.5) and the next one
-// (GFIRST_GPU_Kernels.cu:260-263, 270-279 set linear filter + clamp).
-__device__ __forceinline__ void axis(float c, int n, int& i0, int& i1, float& a) {
-  float t = c - 0.5f;
-  float f = floorf(t);
-  int i = (int)f;
-  a = t - f;
-  i0 = min(max(i, 0), n - 1);
-  i1 = min(max(i + 1, 0), n - 1);
-}
-
-// Trilinear sample of the image at an unnormalized (x, y, z): replaces
-// tex3D(imgEstTex, x, y, z) (GFIRST_GPU_Kernels.cu:97).
-__device__ __forceinline__ float tex3(const float* __restrict__ v, int W, int H, int D, float x,
-                                      float y, float z) {
-  int x0, x1, y0, y1, z0, z1;
-  float ax, ay, az;
-  axis(x, W, x0, x1, ax);
-  axis(y, H, y0, y1, ay);
-  axis(z, D, z0, z1, az);
-  const float* p0 = v + (size_t)z0 * W * H;
-  const float* p1 = v + (size_t)z1 * W * H;
-  float b0 = lerp1(lerp1(p0[y0 * W + x0], p0[y0 * W + x1], ax),
-                   lerp1(p0[y1 * W + x0], p0[y1 * W + x1], ax), ay);
-  float b1 = lerp1(lerp1(p1[y0 * W + x0], p1[y0 * W + x1], ax),
-                   lerp1(p1[y1 * W + x0], p1[y1 * W + x1], ax), ay);
-  return lerp1(b0, b1, az);
-}
-
-template <int S>
-__device__ __forceinline__ float load_store(const void* p, size_t i) {
-  if constexpr (S == GFR_STORE_F32) {
-    return ((const float*)p)[i];
-  } else if constexpr (S == GFR_STORE_F16) {
-    return (float)((const _Float16*)p)[i];
-  } else {
-    uint32_t b = (uint32_t)((const uint16_t*)p)[i] << 16;
-    return __uint_as_float(b);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Forward projection (+ optional ratio and FP64 log-likelihood epilogue).
-// One thread per LOR (view = (segment, angle), plane, radial bin): the LOR
-// sum of RES+1 trilinear samples of FORWPROJ_K (GFIRST_GPU_Kernels.cu:73-98),
-// then the ratio of GFIRST.cu:415-430 for the same bin.
-// Block = 64 radial bins x 4 sinograms; grid.z = angles of the subset.
-// ---------------------------------------------------------------------------
-template <int S, bool RATIO, bool LLH>
-__global__ __launch_bounds__(256) void k_forward(DevGeom G, const float* __restrict__ img,
-                                                 float* __restrict__ proj,
-                                                 const void* __restrict__ y,
-                                                 const void* __restrict__ bg, float yscale,
-                                                 float* __restrict__ corr,
-                                                 double* __restrict__ llh_part, int subset,
-                                                 int nsub) {
-  const int r = blockIdx.x * kWave + threadIdx.x;
-  const int s = blockIdx.y * 4 + threadIdx.y;
-  const int it = subset + (int)blockIdx.z * nsub;
-  double ll = 0.0;
-  if (r < G.nrad && s < G.nsinogs) {
-    const int2 sp = G.sino[s];
-    const SegEntry se = G.seg[sp.x];
-    const float st = G.sin_t[it], ct = G.cos_t[it];
-    const float dx = st, dy = -ct, dz = se.dzf;
-    const float zc = (float)sp.y + 0.5f;
-    const float x0 = G.fxf * ((float)r - G.RADM);
-    const float xc = fmaf(x0, ct, G.RESM) + 0.5f;
-    const float yc = fmaf(x0, st, G.RESM) + 0.5f;
-    float acc = 0.f;
-    for (int l = G.l0; l <= G.l1; l++) {
-      const float fl = (float)l;
-      acc += tex3(img, G.res, G.res, G.nzs, fmaf(fl, dx, xc), fmaf(fl, dy, yc), fmaf(fl, dz, zc));
-    }
-    const size_t bin = ((size_t)s * G.nang + it) * G.nrad + r;
-    if (proj) proj[bin] = acc;
-    if constexpr (RATIO) {
-      const float yv = yscale * load_store<S>(y, bin);
-      float bv = bg ? yscale * load_store<S>(bg, bin) : 0.f;
-      if (bv < 0.f) bv = 0.f;
-      const float den = acc + bv;
-      corr[bin] = (den >= 0.01f) ? yv / den : -1000.f;
-      if constexpr (LLH) {
-        if (den > 0.f) ll = (double)yv * log((double)den) - (double)den;
-      }
-    }
-  }
-  if constexpr (LLH) {
-    __shared__ double red[256];
-    const int t = threadIdx.y * kWave + threadIdx.x;
-    red[t] = ll;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (t < w) red[t] += red[t + w];
-      __syncthreads();
-    }
-    if (t == 0) {
-      const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-      llh_part[b] = red[0];
-    }
-  }
-}
 
 // Ratio alone (GFIRST.cu:415-430) over the subset's bins; the background is
 // in storage format S.
 template <int S>
 __global__ void k_ratio(DevGeom G, const float* __restrict__ proj, const float* __restrict__ y,
                         const void* __restrict__ bg, float yscale, float* __restrict__ corr,
-                        int subset, int nsub) {
+                        float2* __restrict__ corr2, const uint8_t* __restrict__ mask, int subset,
+                        int nsub) {
   const int r = blockIdx.x * kWave + threadIdx.x;
   const int s = blockIdx.y * 4 + threadIdx.y;
   const int it = subset + (int)blockIdx.z * nsub;
@@ -137,69 +29,9 @@ __global__ void k_ratio(DevGeom G, const float* __restrict__ proj, const float* 
   float bv = bg ? yscale * load_store<S>(bg, bin) : 0.f;
   if (bv < 0.f) bv = 0.f;
   const float den = p + bv;
-  corr[bin] = (den >= 0.01f) ? yv / den : -1000.f;
-}
-
-// ---------------------------------------------------------------------------
-// Backprojection: one thread per voxel, views in the reference's order
-// (segment-major, then angle; GFIRST.cu:439-465). Per view: the rotation and
-// bilinear fetch of BACKWPROJ_K (GFIRST_GPU_Kernels.cu:136-160) from the
-// sinogram rows of the view; planes outside [zmin, NZS-zmin) read -1000 as
-// the host's CORR_PART/CALIB_PART fill does (GFIRST.cu:447).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_back(DevGeom G, const float* __restrict__ corr,
-                                              const uint8_t* __restrict__ mask,
-                                              float* __restrict__ numer,
-                                              float* __restrict__ denom, int subset, int nsub,
-                                              int nav) {
-  const int ix = blockIdx.x * kWave + threadIdx.x;
-  const int iy = blockIdx.y * 4 + threadIdx.y;
-  const int iz = blockIdx.z;
-  if (ix >= G.res || iy >= G.res) return;
-  const float xv = (float)ix - G.RESM, yv = (float)iy - G.RESM;
-  const size_t rowstride = (size_t)G.nang * G.nrad;
-  float n = 0.f, d = 0.f;
-  for (int si = 0; si < G.nseg; si++) {
-    const SegEntry se = G.seg[si];
-    const int zm = se.zmin;
-    const int zhi = G.nzs - zm;
-    for (int a = 0; a < nav; a++) {
-      const int it = subset + a * nsub;
-      const float st = G.sin_t[it], ct = G.cos_t[it];
-      const float xr = fmaf(yv, st, xv * ct);
-      const float yr = fmaf(yv, ct, -(xv * st));
-      const float u = fmaf(xr, G.fxb, G.RADM) + 0.5f;
-      const float w = fmaf(yr, se.fzb, (float)iz) + 0.5f;
-      float av = 0.f, bv = 0.f;
-      if (w >= (float)zm && w < (float)zhi) {
-        int r0, r1, p0, p1;
-        float au, aw;
-        axis(u, G.nrad, r0, r1, au);
-        axis(w, G.nzs, p0, p1, aw);
-        const size_t rb = ((size_t)se.nza * G.nang + it) * G.nrad;
-        const bool v0 = (p0 >= zm && p0 < zhi), v1 = (p1 >= zm && p1 < zhi);
-        const size_t o0 = rb + (size_t)(p0 - zm) * rowstride;
-        const size_t o1 = rb + (size_t)(p1 - zm) * rowstride;
-        const float c00 = v0 ? corr[o0 + r0] : -1000.f, c01 = v0 ? corr[o0 + r1] : -1000.f;
-        const float c10 = v1 ? corr[o1 + r0] : -1000.f, c11 = v1 ? corr[o1 + r1] : -1000.f;
-        const float k00 = (v0 && mask[o0 + r0]) ? 1.f : -1000.f;
-        const float k01 = (v0 && mask[o0 + r1]) ? 1.f : -1000.f;
-        const float k10 = (v1 && mask[o1 + r0]) ? 1.f : -1000.f;
-        const float k11 = (v1 && mask[o1 + r1]) ? 1.f : -1000.f;
-        av = lerp1(lerp1(c00, c01, au), lerp1(c10, c11, au), aw);
-        bv = lerp1(lerp1(k00, k01, au), lerp1(k10, k11, au), aw);
-      }
-      if (bv < 0.f || av < 0.f) {
-        av = 0.f;
-        bv = 0.f;
-      }
-      n += av;
-      d += bv;
-    }
-  }
-  const size_t iV = ((size_t)iz * G.res + iy) * G.res + ix;
-  numer[iV] = n;
-  denom[iV] = d;
+  const float c = (den >= 0.01f) ? yv / den : -1000.f;
+  if (corr) corr[bin] = c;
+  if (corr2) store_packed(G, corr2, blockIdx.z, s, r, make_float2(c, mask[bin] ? 1.f : -1000.f));
 }
 
 // ---------------------------------------------------------------------------
@@ -220,34 +52,51 @@ struct SrcRatio {  // divVxVGPU (GFIRST_GPU_Kernels.cu:173-182)
   }
 };
 
+// One block per (plane, kBlurRows output rows): the input rows y-4..y+kBlurRows+3
+// are loaded once into LDS (zero outside the plane), y-filtered in LDS, then
+// x-filtered; the SRC functor is evaluated once per loaded element.
+constexpr int kBlurRows = 16;
+
 template <class Src>
 __global__ __launch_bounds__(256) void k_blur_xy(int res, int nzs, Taps9 t, Src src,
                                                  float* __restrict__ out) {
-  const int x = blockIdx.x * kWave + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z;
-  if (x >= res || y >= res) return;
+  extern __shared__ float lds[];
+  float* in = lds;                           // (kBlurRows + 8) x res
+  float* yc = lds + (kBlurRows + 8) * res;   // kBlurRows x res
+  const int z = blockIdx.y;
+  const int y0 = blockIdx.x * kBlurRows;
+  const int nrows = min(kBlurRows, res - y0);
   const size_t pb = (size_t)z * res * res;
-  float o;
-  if (x > 3 && x < res - 4) {
-    float acc = 0.f;
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-      const int xx = x + k - 4;
-      float yc = 0.f;
-#pragma unroll
-      for (int j = 0; j < 9; j++) {
-        const int yy = y + j - 4;
-        const float v = (yy >= 0 && yy < res) ? src(pb + (size_t)yy * res + xx) : 0.f;
-        yc = (j == 0) ? v * t.g[0] : fmaf(v, t.g[j], yc);
-      }
-      acc = (k == 0) ? yc * t.g[0] : fmaf(yc, t.g[k], acc);
-    }
-    o = acc;
-  } else {
-    o = src(pb + (size_t)y * res + x);
+  const int tid = threadIdx.x;
+  const int nin = (nrows + 8) * res;
+  for (int i = tid; i < nin; i += 256) {
+    const int row = i / res, col = i - row * res;
+    const int y = y0 + row - 4;
+    in[i] = (y >= 0 && y < res) ? src(pb + (size_t)y * res + col) : 0.f;
   }
-  out[pb + (size_t)y * res + x] = o;
+  __syncthreads();
+  const int nout = nrows * res;
+  for (int i = tid; i < nout; i += 256) {
+    const int row = i / res, col = i - row * res;
+    float acc = in[row * res + col] * t.g[0];
+#pragma unroll
+    for (int j = 1; j < 9; j++) acc = fmaf(in[(row + j) * res + col], t.g[j], acc);
+    yc[i] = acc;
+  }
+  __syncthreads();
+  for (int i = tid; i < nout; i += 256) {
+    const int row = i / res, x = i - row * res;
+    float o;
+    if (x > 3 && x < res - 4) {
+      const float* yr = yc + row * res + x - 4;
+      o = yr[0] * t.g[0];
+#pragma unroll
+      for (int k = 1; k < 9; k++) o = fmaf(yr[k], t.g[k], o);
+    } else {
+      o = in[(row + 4) * res + x];  // raw input at the x borders
+    }
+    out[pb + (size_t)(y0 + row) * res + x] = o;
+  }
 }
 
 // Gz with the reference's index shift, multiply, slice copies
@@ -511,56 +360,34 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_blur_xy(const DevGeom& G, const Taps9& t, const float* in, float* out,
-                          hipStream_t st) {
-  hipLaunchKernelGGL(k_blur_xy<SrcPlain>, grid_img(G, G.nzs), dim3(kWave, 4), 0, st, G.res,
-                     G.nzs, t, SrcPlain{in}, out);
+template <class Src>
+hipError_t blur_launch(const DevGeom& G, const Taps9& t, Src src, float* out, hipStream_t st) {
+  const size_t lds = sizeof(float) * (size_t)(2 * kBlurRows + 8) * G.res;
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  dim3 grid((G.res + kBlurRows - 1) / kBlurRows, G.nzs);
+  hipLaunchKernelGGL(k_blur_xy<Src>, grid, dim3(256), lds, st, G.res, G.nzs, t, src, out);
   return hipGetLastError();
 }
 
-hipError_t launch_forward(const DevGeom& G, const float* img, float* proj, const void* y,
-                          const void* bg, int storage, float yscale, float* corr,
-                          double* llh_part, int subset, int nsub, hipStream_t st) {
-  const int nav = nav_of(G, subset, nsub);
-  dim3 grid((G.nrad + kWave - 1) / kWave, (G.nsinogs + 3) / 4, nav);
-  dim3 blk(kWave, 4);
-  if (!corr) {
-    hipLaunchKernelGGL((k_forward<GFR_STORE_F32, false, false>), grid, blk, 0, st, G, img, proj,
-                       nullptr, nullptr, 1.f, nullptr, nullptr, subset, nsub);
-    return hipGetLastError();
-  }
-#define GFR_FWD(S)                                                                              \
-  if (llh_part)                                                                                 \
-    hipLaunchKernelGGL((k_forward<S, true, true>), grid, blk, 0, st, G, img, proj, y, bg,      \
-                       yscale, corr, llh_part, subset, nsub);                                   \
-  else                                                                                          \
-    hipLaunchKernelGGL((k_forward<S, true, false>), grid, blk, 0, st, G, img, proj, y, bg,     \
-                       yscale, corr, nullptr, subset, nsub);
-  if (storage == GFR_STORE_F16) {
-    GFR_FWD(GFR_STORE_F16)
-  } else if (storage == GFR_STORE_BF16) {
-    GFR_FWD(GFR_STORE_BF16)
-  } else {
-    GFR_FWD(GFR_STORE_F32)
-  }
-#undef GFR_FWD
-  return hipGetLastError();
+hipError_t launch_blur_xy(const DevGeom& G, const Taps9& t, const float* in, float* out,
+                          hipStream_t st) {
+  return blur_launch(G, t, SrcPlain{in}, out, st);
 }
 
 hipError_t launch_ratio(const DevGeom& G, const float* proj, const float* y, const void* bg,
-                        int storage, float yscale, float* corr, int subset, int nsub,
-                        hipStream_t st) {
+                        int storage, float yscale, float* corr, float2* corr2,
+                        const uint8_t* mask, int subset, int nsub, hipStream_t st) {
   const int nav = nav_of(G, subset, nsub);
   dim3 grid((G.nrad + kWave - 1) / kWave, (G.nsinogs + 3) / 4, nav);
   if (storage == GFR_STORE_F16)
     hipLaunchKernelGGL(k_ratio<GFR_STORE_F16>, grid, dim3(kWave, 4), 0, st, G, proj, y, bg,
-                       yscale, corr, subset, nsub);
+                       yscale, corr, corr2, mask, subset, nsub);
   else if (storage == GFR_STORE_BF16)
     hipLaunchKernelGGL(k_ratio<GFR_STORE_BF16>, grid, dim3(kWave, 4), 0, st, G, proj, y, bg,
-                       yscale, corr, subset, nsub);
+                       yscale, corr, corr2, mask, subset, nsub);
   else
     hipLaunchKernelGGL(k_ratio<GFR_STORE_F32>, grid, dim3(kWave, 4), 0, st, G, proj, y, bg,
-                       yscale, corr, subset, nsub);
+                       yscale, corr, corr2, mask, subset, nsub);
   return hipGetLastError();
 }
 
@@ -572,19 +399,9 @@ hipError_t launch_calib_mask(const DevGeom& G, const float* pb, const float* nor
   return hipGetLastError();
 }
 
-hipError_t launch_back(const DevGeom& G, const float* corr, const uint8_t* mask, float* numer,
-                       float* denom, int subset, int nsub, hipStream_t st) {
-  const int nav = nav_of(G, subset, nsub);
-  hipLaunchKernelGGL(k_back, grid_img(G, G.nzs), dim3(kWave, 4), 0, st, G, corr, mask, numer,
-                     denom, subset, nsub, nav);
-  return hipGetLastError();
-}
-
 hipError_t launch_update(const DevGeom& G, const Taps9& t9, const Taps5& t5, const float* numer,
                          const float* denom, float* x, float* work, hipStream_t st) {
-  hipLaunchKernelGGL(k_blur_xy<SrcRatio>, grid_img(G, G.nzs), dim3(kWave, 4), 0, st, G.res,
-                     G.nzs, t9, SrcRatio{numer, denom}, work);
-  hipError_t e = hipGetLastError();
+  hipError_t e = blur_launch(G, t9, SrcRatio{numer, denom}, work, st);
   if (e != hipSuccess) return e;
   if (G.nzs >= 3)
     hipLaunchKernelGGL(k_gz_mul, grid_img(G, G.nzs - 2), dim3(kWave, 4), 0, st, G.res, G.nzs, t5,

@@ -1,0 +1,375 @@
+// gfr_project.hip — the two projectors of the OSEM path on gfx950.
+//
+// Forward projection (FORWPROJ_K, GFIRST_GPU_Kernels.cu:40-102): a LOR of view
+// (segment, angle theta), plane p, radial bin r sums RES+1 trilinear samples of
+// the blurred image at (xc + l dx, yc + l dy, p + 0.5 + l dz). The x/y sample
+// positions depend only on (theta, r, l); only z depends on the segment and
+// plane. Trilinear = lerp_z(bilinear_xy(plane z0), bilinear_xy(plane z1)), so
+// the kernel pair below computes every bilinear value once per angle:
+//   k_rotate: R[a][l][z][r] = bilinear_xy(image plane z, x(theta,r,l), y(theta,r,l))
+//   k_zsum  : P(s, r) = sum_l lerp(R[a][l][z0][r], R[a][l][z1][r], alpha_z)
+// which is the same arithmetic, operation for operation, as sampling the
+// volume directly (bit-identical results). The z interpolation of sinogram s
+// at sample l, (z0, z1, alpha_z), is a per-geometry table, so a sample costs
+// two coalesced buffer loads with scalar offsets and one lerp.
+//
+// Backprojection (BACKWPROJ_K, GFIRST_GPU_Kernels.cu:109-164): one thread
+// per voxel column and kZB consecutive slices, views in the reference's order
+// (segment-major, angle-minor, GFIRST.cu:439-465). Per view the radial
+// interpolation (r0, r1, alpha_u) is shared by the thread's slices; the
+// kZB+1 planes its slices need are fetched together (one memory round trip
+// per view, row addresses without tests thanks to the padded packed rows)
+// and radially interpolated once each. Each slice keeps its own exactly
+// computed plane index and weight, as the reference's per-voxel sampling.
+#include "gfr_device.h"
+
+namespace gfr {
+
+namespace {
+
+// Raw buffer resource over [p, p + bytes): 32-bit offsets, loads past the end
+// return 0 without touching memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ float ld_f32(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ float2 ld_f2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+constexpr unsigned kOOB = 0xFFFFFFF0u;  // an offset past any buffer: the load returns 0
+
+// ---------------------------------------------------------------------------
+// k_rotate: one thread per (angle a of the chunk, sample l, plane z, bin r).
+// Logical block order (r-block, l-block, z, a) with XCD-contiguous ranges, so
+// the work-groups sharing an image plane share an L2.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_rotate(DevGeom G, const float* __restrict__ img,
+                                                float* __restrict__ R, int subset, int nsub,
+                                                int a0, int nl) {
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const unsigned nrb = (G.nrad + kWave - 1) / kWave, nlb = (nl + 3) / 4;
+  const unsigned rb = b % nrb;
+  unsigned q = b / nrb;
+  const unsigned lb = q % nlb;
+  q /= nlb;
+  const int z = (int)(q % G.nzs);
+  const int a = (int)(q / G.nzs);
+  const int r = rb * kWave + threadIdx.x;
+  const int li = lb * 4 + threadIdx.y;
+  if (r >= G.nrad || li >= nl) return;
+  const int it = subset + (a0 + a) * nsub;
+  const float st = G.sin_t[it], ct = G.cos_t[it];
+  const float x0 = G.fxf * ((float)r - G.RADM);
+  const float xc = fmaf(x0, ct, G.RESM) + 0.5f;
+  const float yc = fmaf(x0, st, G.RESM) + 0.5f;
+  const float fl = (float)(G.l0 + li);
+  const float x = fmaf(fl, st, xc);   // dx = sin(theta)   (GFIRST_GPU_Kernels.cu:81)
+  const float y = fmaf(fl, -ct, yc);  // dy = -cos(theta)  (:82)
+  int x0i, x1i, y0i, y1i;
+  float ax, ay;
+  axis(x, G.res, x0i, x1i, ax);
+  axis(y, G.res, y0i, y1i, ay);
+  const float* p = img + (size_t)z * G.res * G.res;
+  const float v = lerp1(lerp1(p[y0i * G.res + x0i], p[y0i * G.res + x1i], ax),
+                        lerp1(p[y1i * G.res + x0i], p[y1i * G.res + x1i], ax), ay);
+  R[(((size_t)a * nl + li) * G.nzs + z) * G.nrad + r] = v;
+}
+
+// Where the forward kernel writes.
+struct FwdArgs {
+  float* proj;           // full sinogram layout, or NULL
+  float* corr;           // full layout ratio, or NULL
+  float2* corr2;         // packed [angle of subset][s][r] {ratio, calib}, or NULL
+  const uint8_t* mask;   // calibration mask (full layout) for corr2
+  const void* y;         // measured prompts (storage format), unscaled
+  const void* bg;        // background or NULL
+  float yscale;          // RES (GFIRST.cu:224-260)
+  double* llh_part;      // per-block FP64 partial log-likelihoods, or NULL
+};
+
+// ---------------------------------------------------------------------------
+// k_zsum: one thread per LOR (angle a, sinogram s, bin r); a wave is 64 bins
+// of one sinogram, so the z interpolation is wave-uniform (scalar table loads
+// and scalar buffer offsets) and both loads per sample are contiguous rows.
+// Epilogue: ratio (GFIRST.cu:415-430) and the FP64 Poisson log-likelihood
+// partial of the block.
+// ---------------------------------------------------------------------------
+template <int S, bool RATIO, bool LLH>
+__global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict__ R,
+                                              unsigned r_bytes, FwdArgs o, int subset, int nsub,
+                                              int a0, int nl, unsigned llh_off) {
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const unsigned nrb = (G.nrad + kWave - 1) / kWave, nsb = (G.nsinogs + 3) / 4;
+  const unsigned rb = b % nrb;
+  const unsigned q = b / nrb;
+  const unsigned sb = q % nsb;
+  const int a = (int)(q / nsb);
+  const int r = rb * kWave + threadIdx.x;
+  const int s = __builtin_amdgcn_readfirstlane((int)(sb * 4 + threadIdx.y));
+  const int ag = a0 + a;
+  const int it = subset + ag * nsub;
+  double ll = 0.0;
+  if (r < G.nrad && s < G.nsinogs) {
+    const __amdgpu_buffer_rsrc_t rr = rsrc(R, r_bytes);
+    const unsigned rowb = (unsigned)G.nrad * 4u;
+    const unsigned planeb = rowb * (unsigned)G.nzs;
+    const unsigned voff = (unsigned)r * 4u;
+    const uint2* zt = G.ztab + (size_t)s * nl;
+    unsigned soff = (unsigned)a * (unsigned)nl * planeb;
+    float acc = 0.f;
+    for (int li = 0; li < nl; li++, soff += planeb) {
+      const uint2 e = zt[li];
+      const float v0 = ld_f32(rr, voff, soff + (e.x & 0xFFFFu) * rowb);
+      const float v1 = ld_f32(rr, voff, soff + (e.x >> 16) * rowb);
+      acc += lerp1(v0, v1, __uint_as_float(e.y));
+    }
+    const size_t bin = ((size_t)s * G.nang + it) * G.nrad + r;
+    if (o.proj) o.proj[bin] = acc;
+    if constexpr (RATIO) {
+      const float yv = o.yscale * load_store<S>(o.y, bin);
+      float bv = o.bg ? o.yscale * load_store<S>(o.bg, bin) : 0.f;
+      if (bv < 0.f) bv = 0.f;
+      const float den = acc + bv;
+      const float c = (den >= 0.01f) ? yv / den : -1000.f;
+      if (o.corr) o.corr[bin] = c;
+      if (o.corr2) store_packed(G, o.corr2, ag, s, r, make_float2(c, o.mask[bin] ? 1.f : -1000.f));
+      if constexpr (LLH) {
+        if (den > 0.f) ll = (double)yv * log((double)den) - (double)den;
+      }
+    }
+  }
+  if constexpr (LLH) {
+    __shared__ double red[256];
+    const int t = threadIdx.y * kWave + threadIdx.x;
+    red[t] = ll;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (t < w) red[t] += red[t + w];
+      __syncthreads();
+    }
+    if (t == 0) o.llh_part[llh_off + blockIdx.x] = red[0];
+  }
+}
+
+// Full-layout ratio + byte mask -> packed {ratio, calib} rows of a subset.
+__global__ void k_pack(DevGeom G, const float* __restrict__ corr, const uint8_t* __restrict__ mask,
+                       float2* __restrict__ c2, int subset, int nsub) {
+  const int r = blockIdx.x * kWave + threadIdx.x;
+  const int s = blockIdx.y * 4 + threadIdx.y;
+  const int a = blockIdx.z;
+  if (r >= G.nrad || s >= G.nsinogs) return;
+  const size_t bin = ((size_t)s * G.nang + subset + a * nsub) * G.nrad + r;
+  store_packed(G, c2, a, s, r, make_float2(corr[bin], mask[bin] ? 1.f : -1000.f));
+}
+
+constexpr int kZB = 8;  // slices per thread
+
+// Block: 4 waves, each an 8x8 tile of voxel columns (16x16 per block) times
+// kZB slices. Across an 8x8 tile the z tilt of a view moves by a few planes
+// only, so a wave with no slice inside the view's plane range skips the view
+// (wave-uniform test); the tile's radial bins are a few neighbouring bins, so
+// row loads stay within a few cache lines.
+__global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restrict__ c2,
+                                              unsigned c2_bytes, float* __restrict__ numer,
+                                              float* __restrict__ denom, int subset, int nsub,
+                                              int nav) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ix = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
+  const int iy = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
+  const int zb = blockIdx.z * kZB;
+  const bool act = ix < G.res && iy < G.res;
+  const __amdgpu_buffer_rsrc_t rc2 = rsrc(c2, c2_bytes);
+  const float xv = (float)ix - G.RESM, yv = (float)iy - G.RESM;
+  const unsigned rowb = (unsigned)G.nrad * 8u;
+  const unsigned nr = (unsigned)(G.nsinogs + 2 * G.nseg);
+  const float fz0 = (float)zb, fzl = (float)(zb + kZB - 1);
+  float n[kZB], d[kZB];
+#pragma unroll
+  for (int j = 0; j < kZB; j++) n[j] = d[j] = 0.f;
+  for (int si = 0; si < G.nseg; si++) {
+    const SegEntry se = G.seg[si];
+    const int zm = se.zmin;
+    const int zhi = G.nzs - zm;
+    const float fzm = (float)zm, fzhi = (float)zhi;
+    // row of plane p of this segment within an angle: segrow + p (planes
+    // zm-1 .. zhi are the padded rows)
+    const unsigned segrow = (unsigned)(se.nza + 2 * si + 1 - zm);
+    for (int a = 0; a < nav; a++) {
+      const int it = subset + a * nsub;
+      const float st = G.sin_t[it], ct = G.cos_t[it];
+      const float yr = fmaf(yv, ct, -(xv * st));
+      const float w0 = fmaf(yr, se.fzb, fz0) + 0.5f;
+      const float wl = fmaf(yr, se.fzb, fzl) + 0.5f;
+      const bool any = act && wl >= fzm && w0 < fzhi;
+      if (__builtin_amdgcn_ballot_w64(any) == 0) continue;
+      const float xr = fmaf(yv, st, xv * ct);
+      const float u = fmaf(xr, G.fxb, G.RADM) + 0.5f;
+      int r0, r1;
+      float au;
+      axis(u, G.nrad, r0, r1, au);
+      const unsigned o0 = (unsigned)r0 * 8u, o1 = (unsigned)r1 * 8u;
+      const unsigned vrow = (unsigned)a * nr + segrow;
+      float al[kZB];
+      int ii[kZB];
+      bool inr[kZB];
+#pragma unroll
+      for (int j = 0; j < kZB; j++) {
+        const float w = (j == 0) ? w0 : (j == kZB - 1) ? wl : fmaf(yr, se.fzb, (float)(zb + j)) + 0.5f;
+        inr[j] = act && (zb + j < G.nzs) && w >= fzm && w < fzhi;
+        const float t = w - 0.5f;
+        const float f = floorf(t);
+        al[j] = t - f;
+        ii[j] = (int)f;
+      }
+      bool reg = true;
+#pragma unroll
+      for (int j = 1; j < kZB; j++) reg = reg && (!inr[j] || ii[j] == ii[0] + j);
+      float av[kZB], bv[kZB];
+      if (reg) {
+        // planes ii0 + k, k = 0..kZB: slice j blends planes j and j+1
+        const unsigned rb0 = (vrow + (unsigned)ii[0]) * rowb;
+        float c[kZB + 1], q[kZB + 1];
+        float2 v0[kZB + 1], v1[kZB + 1];
+#pragma unroll
+        for (int k = 0; k <= kZB; k++) {
+          v0[k] = ld_f2(rc2, rb0 + o0, (unsigned)k * rowb);
+          v1[k] = ld_f2(rc2, rb0 + o1, (unsigned)k * rowb);
+        }
+#pragma unroll
+        for (int k = 0; k <= kZB; k++) {
+          c[k] = lerp1(v0[k].x, v1[k].x, au);
+          q[k] = lerp1(v0[k].y, v1[k].y, au);
+        }
+#pragma unroll
+        for (int j = 0; j < kZB; j++) {
+          av[j] = inr[j] ? lerp1(c[j], c[j + 1], al[j]) : 0.f;
+          bv[j] = inr[j] ? lerp1(q[j], q[j + 1], al[j]) : 0.f;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < kZB; j++) {
+          av[j] = 0.f;
+          bv[j] = 0.f;
+          if (inr[j]) {
+            const unsigned ra = (vrow + (unsigned)ii[j]) * rowb;
+            const float2 w00 = ld_f2(rc2, ra + o0, 0), w01 = ld_f2(rc2, ra + o1, 0);
+            const float2 w10 = ld_f2(rc2, ra + o0, rowb), w11 = ld_f2(rc2, ra + o1, rowb);
+            av[j] = lerp1(lerp1(w00.x, w01.x, au), lerp1(w10.x, w11.x, au), al[j]);
+            bv[j] = lerp1(lerp1(w00.y, w01.y, au), lerp1(w10.y, w11.y, au), al[j]);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kZB; j++) {
+        float A = av[j], B = bv[j];
+        if (B < 0.f || A < 0.f) {  // gaps (GFIRST_GPU_Kernels.cu:157)
+          A = 0.f;
+          B = 0.f;
+        }
+        n[j] += A;
+        d[j] += B;
+      }
+    }
+  }
+  if (!act) return;
+#pragma unroll
+  for (int j = 0; j < kZB; j++) {
+    const int iz = zb + j;
+    if (iz < G.nzs) {
+      const size_t iV = ((size_t)iz * G.res + iy) * G.res + ix;
+      numer[iV] = n[j];
+      denom[iV] = d[j];
+    }
+  }
+}
+
+inline int nav_of(const DevGeom& G, int subset, int nsub) {
+  return (G.nang - subset + nsub - 1) / nsub;
+}
+
+}  // namespace
+
+size_t rotate_floats_per_angle(const DevGeom& G) {
+  return (size_t)(G.l1 - G.l0 + 1) * G.nzs * G.nrad;
+}
+
+int forward_llh_blocks(const DevGeom& G, int subset, int nsub) {
+  return ((G.nrad + kWave - 1) / kWave) * ((G.nsinogs + 3) / 4) * nav_of(G, subset, nsub);
+}
+
+hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r_cap_floats,
+                          const FwdOut& out, int subset, int nsub, hipStream_t st) {
+  const int nav = nav_of(G, subset, nsub);
+  const int nl = G.l1 - G.l0 + 1;
+  const size_t per = rotate_floats_per_angle(G);
+  int chunk = (int)(r_cap_floats / per);
+  if (chunk < 1) return hipErrorInvalidValue;
+  if (chunk > nav) chunk = nav;
+  if ((size_t)chunk * per * sizeof(float) >= 0xFFFFFFF0u) chunk = (int)(0xFFFFFFF0u / (per * 4));
+  if (chunk < 1) return hipErrorInvalidValue;
+  FwdArgs o{out.proj, out.corr, out.corr2, out.mask, out.y, out.bg, out.yscale, out.llh_part};
+  const bool ratio = out.corr || out.corr2;
+  const unsigned nrb = (G.nrad + kWave - 1) / kWave, nsb = (G.nsinogs + 3) / 4;
+  for (int a0 = 0; a0 < nav; a0 += chunk) {
+    const int na = min(chunk, nav - a0);
+    const unsigned rbytes = (unsigned)((size_t)na * per * sizeof(float));
+    const unsigned grot = nrb * ((nl + 3) / 4) * G.nzs * na;
+    hipLaunchKernelGGL(k_rotate, dim3(grot), dim3(kWave, 4), 0, st, G, img, R, subset, nsub, a0,
+                       nl);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const unsigned gz = nrb * nsb * na;
+    const unsigned off = nrb * nsb * a0;
+    if (!ratio) {
+      hipLaunchKernelGGL((k_zsum<GFR_STORE_F32, false, false>), dim3(gz), dim3(kWave, 4), 0, st, G,
+                         R, rbytes, o, subset, nsub, a0, nl, off);
+    } else {
+#define GFR_ZS(S)                                                                           \
+  if (out.llh_part)                                                                         \
+    hipLaunchKernelGGL((k_zsum<S, true, true>), dim3(gz), dim3(kWave, 4), 0, st, G, R, rbytes, \
+                       o, subset, nsub, a0, nl, off);                                       \
+  else                                                                                      \
+    hipLaunchKernelGGL((k_zsum<S, true, false>), dim3(gz), dim3(kWave, 4), 0, st, G, R,    \
+                       rbytes, o, subset, nsub, a0, nl, off);
+      if (out.storage == GFR_STORE_F16) {
+        GFR_ZS(GFR_STORE_F16)
+      } else if (out.storage == GFR_STORE_BF16) {
+        GFR_ZS(GFR_STORE_BF16)
+      } else {
+        GFR_ZS(GFR_STORE_F32)
+      }
+#undef GFR_ZS
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+size_t packed_bytes(const DevGeom& G, int subset, int nsub) {
+  return sizeof(float2) * (size_t)nav_of(G, subset, nsub) * (G.nsinogs + 2 * (size_t)G.nseg) *
+         G.nrad;
+}
+
+hipError_t launch_pack(const DevGeom& G, const float* corr, const uint8_t* mask, float2* c2,
+                       int subset, int nsub, hipStream_t st) {
+  dim3 grid((G.nrad + kWave - 1) / kWave, (G.nsinogs + 3) / 4, nav_of(G, subset, nsub));
+  hipLaunchKernelGGL(k_pack, grid, dim3(kWave, 4), 0, st, G, corr, mask, c2, subset, nsub);
+  return hipGetLastError();
+}
+
+hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* numer, float* denom,
+                              int subset, int nsub, hipStream_t st) {
+  const int nav = nav_of(G, subset, nsub);
+  const size_t bytes = packed_bytes(G, subset, nsub);
+  if (bytes >= 0xFFFFFFF0u) return hipErrorInvalidValue;
+  dim3 grid((G.res + 15) / 16, (G.res + 15) / 16, (G.nzs + kZB - 1) / kZB);
+  hipLaunchKernelGGL(k_back, grid, dim3(256), 0, st, G, corr2, (unsigned)bytes, numer, denom,
+                     subset, nsub, nav);
+  return hipGetLastError();
+}
+
+}  // namespace gfr

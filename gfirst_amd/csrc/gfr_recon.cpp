@@ -8,6 +8,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "gfr_internal.h"
@@ -25,7 +26,10 @@ struct gfr_recon {
   float *xD = nullptr, *xT = nullptr, *xb = nullptr, *num = nullptr, *den = nullptr,
         *work = nullptr, *jo = nullptr;
   // sinogram-sized
-  float *corr = nullptr, *pb = nullptr, *ptc = nullptr, *dv = nullptr;
+  float *pb = nullptr, *ptc = nullptr, *dv = nullptr;
+  float2* corr2 = nullptr;  // {ratio, calib} of the current subset, [angle][s][r]
+  float* R = nullptr;       // rotation stage of the forward projector
+  size_t r_floats = 0;
   void *yD = nullptr, *bD = nullptr, *yT = nullptr, *bT = nullptr;
   uint8_t* mask = nullptr;
   gfr_vlor* vl = nullptr;
@@ -56,7 +60,7 @@ int llh_blocks(const GeomHost& h, int nsub) {
 }
 
 struct Plan {
-  size_t img, sino_f32, sino_store, mask, llh;
+  size_t img, sino_f32, sino_store, mask, llh, corr2, rot;
   int nimg, nsino_f32, nsino_store;
 };
 
@@ -73,16 +77,20 @@ Plan plan_of(const GeomHost& h, const gfr_recon_params* p, bool bgD, bool bgT) {
   pl.sino_store = store_bytes(p->storage) * (size_t)h.info.ndata;
   pl.mask = (size_t)h.info.ndata;
   pl.llh = sizeof(double) * (size_t)llh_blocks(h, p->nsub) + 64;
+  const size_t nav = (size_t)(h.g.nang + p->nsub - 1) / p->nsub;
+  pl.corr2 = sizeof(float2) * nav * ((size_t)h.info.nsinogs + 2 * (size_t)h.g.nseg) * h.g.nrad;
+  pl.rot = sizeof(float) * (size_t)(h.info.lambda1 - h.info.lambda0 + 1) * h.g.nzs * h.g.nrad *
+           std::min(nav, (size_t)kRotChunk);
   const bool two = p->mode >= 2;
   pl.nimg = 6 + (two ? 2 : 0);                      // xD num den work xb (+xT jo)
-  pl.nsino_f32 = 2 + (p->mode == 3 ? 2 : 0);        // corr pb (+ptc dv)
+  pl.nsino_f32 = 1 + (p->mode == 3 ? 2 : 0);        // pb (+ptc dv)
   pl.nsino_store = 1 + (bgD ? 1 : 0) + (two ? 1 + (bgT ? 1 : 0) : 0);
   return pl;
 }
 
 size_t plan_bytes(const Plan& pl) {
   return pl.nimg * pl.img + pl.nsino_f32 * pl.sino_f32 + pl.nsino_store * pl.sino_store +
-         pl.mask + pl.llh;
+         pl.mask + pl.llh + pl.corr2 + pl.rot;
 }
 
 hipEvent_t take_event(gfr_recon* r) {
@@ -182,7 +190,8 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   };
   bool ok = A((void**)&r->xD, pl.img) && A((void**)&r->num, pl.img) && A((void**)&r->den, pl.img) &&
             A((void**)&r->work, pl.img) && A((void**)&r->xb, pl.img) &&
-            A((void**)&r->corr, pl.sino_f32) && A((void**)&r->pb, pl.sino_f32) &&
+            A((void**)&r->corr2, pl.corr2) && A((void**)&r->R, pl.rot) &&
+            A((void**)&r->pb, pl.sino_f32) &&
             A((void**)&r->mask, pl.mask) && A((void**)&r->yD, pl.sino_store) &&
             A((void**)&r->llh_part, pl.llh) && A((void**)&r->d_maxbits, 64) &&
             A((void**)&r->d_llh, 64);
@@ -194,22 +203,26 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   if (!ok) return rfail(r, set_err(ctx, GFR_ENOMEM, "reconstruction allocation failed"));
   hipStream_t st = r->st;
   const DevGeom& G = gd->dev;
+  r->r_floats = pl.rot / sizeof(float);
   hipError_t e = hipSuccess;
-  // Upload a host float array into storage format through the corr buffer.
+  // Host arrays reach the device through a transient staging buffer.
+  float* stage = nullptr;
+  if (hipMalloc((void**)&stage, pl.sino_f32) != hipSuccess)
+    return rfail(r, set_err(ctx, GFR_ENOMEM, "reconstruction staging allocation failed"));
   auto upload = [&](const float* host, void* dst) -> hipError_t {
-    hipError_t ee = hipMemcpyAsync(r->corr, host, pl.sino_f32, hipMemcpyHostToDevice, st);
-    if (ee == hipSuccess) ee = launch_convert_store(r->corr, dst, p->storage, nd, st);
+    hipError_t ee = hipMemcpyAsync(stage, host, pl.sino_f32, hipMemcpyHostToDevice, st);
+    if (ee == hipSuccess) ee = launch_convert_store(stage, dst, p->storage, nd, st);
     if (ee == hipSuccess) ee = hipStreamSynchronize(st);
     return ee;
   };
   // Mask source: projection of the unblurred uniform cylinder (GFIRST.cu:155-181)
   // and the calibration mask (GFIRST.cu:453-458) from the normalization.
   e = launch_fill_cylinder(G, 1.f, r->xb, st);
-  if (e == hipSuccess)
-    e = launch_forward(G, r->xb, r->pb, nullptr, nullptr, GFR_STORE_F32, 1.f, nullptr, nullptr, 0,
-                       1, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(r->corr, norm, pl.sino_f32, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = launch_calib_mask(G, r->pb, r->corr, r->mask, st);
+  FwdOut fo;
+  fo.proj = r->pb;
+  if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fo, 0, 1, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(stage, norm, pl.sino_f32, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_calib_mask(G, r->pb, stage, r->mask, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   // Measured data, stored unscaled; kernels scale by RES on load (GFIRST.cu:224-260).
   if (e == hipSuccess) e = upload(d2, r->yD);
@@ -220,6 +233,8 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   if (e == hipSuccess) e = launch_fill_cylinder(G, 1.f, r->xD, st);
   if (e == hipSuccess && r->ntypes == 2)
     e = launch_fill_cylinder(G, p->mode == 1 ? 0.f : 1.f, r->xT, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(stage);
   if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create"));
   if (p->mode == 3) {
     // BLANKT needs T x RES (GFIRST.cu:225-226, 246): scale a float copy.
@@ -235,9 +250,9 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
     psf_taps(p->psf_fw, t9.g);
     e = launch_zero(r->dv, pl.sino_f32, st);
     if (e == hipSuccess) e = launch_blur_xy(G, t9, r->xT, r->xb, st);
-    if (e == hipSuccess)
-      e = launch_forward(G, r->xb, r->ptc, nullptr, nullptr, GFR_STORE_F32, 1.f, nullptr, nullptr,
-                         0, 1, st);
+    FwdOut fc;
+    fc.proj = r->ptc;
+    if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fc, 0, 1, st);
     if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create vlor cache"));
   }
   e = hipStreamSynchronize(st);
@@ -271,30 +286,38 @@ int gfr_recon_step(gfr_recon* r) {
     // (GFIRST.cu:373-397), ratio against the pseudo-sinogram x RES.
     {
       PhaseMark pm(r, GFR_PHASE_FORWARD);
-      GFR_HIP(ctx, launch_forward(G, r->xb, r->ptc, nullptr, nullptr, GFR_STORE_F32, 1.f, nullptr,
-                                  nullptr, s, S, st));
+      FwdOut fc;
+      fc.proj = r->ptc;
+      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fc, s, S, st));
     }
     PhaseMark pm(r, GFR_PHASE_VLOR);
     int rc = gfr_vlor_assign(ctx, r->vl, r->ptc, r->dv, s, S, st);
     if (rc != GFR_OK) return rc;
-    GFR_HIP(ctx, launch_ratio(G, r->ptc, r->dv, r->bT, p.storage, (float)r->g.res, r->corr, s, S,
-                              st));
+    GFR_HIP(ctx, launch_ratio(G, r->ptc, r->dv, r->bT, p.storage, (float)r->g.res, nullptr,
+                              r->corr2, r->mask, s, S, st));
     r->llh_valid = 0;
   } else {
     const void* y = ty == 0 ? r->yD : r->yT;
     const void* bg = ty == 0 ? r->bD : r->bT;
     {
       PhaseMark pm(r, GFR_PHASE_FORWARD);
-      GFR_HIP(ctx, launch_forward(G, r->xb, nullptr, y, bg, p.storage, (float)r->g.res, r->corr,
-                                  r->llh_part, s, S, st));
+      FwdOut fo;
+      fo.corr2 = r->corr2;
+      fo.mask = r->mask;
+      fo.y = y;
+      fo.bg = bg;
+      fo.storage = p.storage;
+      fo.yscale = (float)r->g.res;
+      fo.llh_part = r->llh_part;
+      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fo, s, S, st));
     }
-    GFR_HIP(ctx, launch_llh_final(r->llh_part, llh_blocks(r->gd->host, S), r->d_llh, st));
+    GFR_HIP(ctx, launch_llh_final(r->llh_part, forward_llh_blocks(G, s, S), r->d_llh, st));
     r->llh_valid = 1;
   }
   // Backprojection of the subset (GFIRST.cu:436-469) and update (:482-502).
   {
     PhaseMark pm(r, GFR_PHASE_BACK);
-    GFR_HIP(ctx, launch_back(G, r->corr, r->mask, r->num, r->den, s, S, st));
+    GFR_HIP(ctx, launch_back_packed(G, r->corr2, r->num, r->den, s, S, st));
   }
   {
     PhaseMark pm(r, GFR_PHASE_UPDATE);
