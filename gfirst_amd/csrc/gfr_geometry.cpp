@@ -102,7 +102,7 @@ int derive_geometry(const gfr_geometry* g, GeomHost* out, std::string* err) {
       const int z1 = i + 1 < 0 ? 0 : (i + 1 > g->nzs - 1 ? g->nzs - 1 : i + 1);
       uint32_t ab;
       memcpy(&ab, &a, 4);
-      h.ztab[(size_t)s * nl + li] = make_uint2((uint32_t)z0 | ((uint32_t)z1 << 16), ab);
+      h.ztab[(size_t)li * acc + s] = make_uint2((uint32_t)z0 | ((uint32_t)z1 << 16), ab);
     }
   }
   return GFR_OK;

@@ -35,7 +35,7 @@ struct DevGeom {
   const float* cos_t;     // [nang]
   const SegEntry* seg;    // [nseg]
   const int2* sino;       // [nsinogs]: (segment index, absolute plane)
-  // [nsinogs][l1-l0+1]: z interpolation of each LOR sample, x = z0 | z1 << 16,
+  // [l1-l0+1][nsinogs]: z interpolation of each LOR sample, x = z0 | z1 << 16,
   // y = bits of alpha_z (GFIRST_GPU_Kernels.cu:88, 96 through the clamp filter)
   const uint2* ztab;
 };

@@ -39,7 +39,6 @@ __device__ __forceinline__ float ld_f32(__amdgpu_buffer_rsrc_t r, unsigned voff,
 __device__ __forceinline__ float2 ld_f2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
-constexpr unsigned kOOB = 0xFFFFFFF0u;  // an offset past any buffer: the load returns 0
 
 // ---------------------------------------------------------------------------
 // k_rotate: one thread per (angle a of the chunk, sample l, plane z, bin r).
@@ -91,59 +90,105 @@ struct FwdArgs {
 };
 
 // ---------------------------------------------------------------------------
-// k_zsum: one thread per LOR (angle a, sinogram s, bin r); a wave is 64 bins
-// of one sinogram, so the z interpolation is wave-uniform (scalar table loads
-// and scalar buffer offsets) and both loads per sample are contiguous rows.
-// Epilogue: ratio (GFIRST.cu:415-430) and the FP64 Poisson log-likelihood
-// partial of the block.
+// k_zsum: one wave per (angle a, kSG consecutive sinograms, 128 radial bins);
+// lane l owns bins 2l and 2l+1 of each sinogram of the group, so a row load is
+// one contiguous 512-byte dwordx2 access. The z interpolation of a sample is
+// wave-uniform (scalar loads of the transposed z table). When the group's
+// samples at l blend planes z0, z0+1, ..., z0+kSG (consecutive planes: the
+// usual case inside a segment), the kSG+1 rows are loaded once for the group;
+// otherwise each LOR loads its own two rows. Every LOR still sums its samples
+// in l order with its own weights. Epilogue: ratio (GFIRST.cu:415-430) and the
+// FP64 Poisson log-likelihood partial of the block.
 // ---------------------------------------------------------------------------
+constexpr int kSG = 8;
+
 template <int S, bool RATIO, bool LLH>
 __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict__ R,
                                               unsigned r_bytes, FwdArgs o, int subset, int nsub,
                                               int a0, int nl, unsigned llh_off) {
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
-  const unsigned nrb = (G.nrad + kWave - 1) / kWave, nsb = (G.nsinogs + 3) / 4;
+  const unsigned nrb = (G.nrad + 127) / 128, ngb = (G.nsinogs + 4 * kSG - 1) / (4 * kSG);
   const unsigned rb = b % nrb;
   const unsigned q = b / nrb;
-  const unsigned sb = q % nsb;
-  const int a = (int)(q / nsb);
-  const int r = rb * kWave + threadIdx.x;
-  const int s = __builtin_amdgcn_readfirstlane((int)(sb * 4 + threadIdx.y));
+  const unsigned gb = q % ngb;
+  const int a = (int)(q / ngb);
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s0 = __builtin_amdgcn_readfirstlane((int)((gb * 4 + wv) * kSG));
+  const int r = (int)rb * 128 + 2 * lane;
   const int ag = a0 + a;
   const int it = subset + ag * nsub;
   double ll = 0.0;
-  if (r < G.nrad && s < G.nsinogs) {
+  if (s0 < G.nsinogs) {
+    const int ns = min(kSG, G.nsinogs - s0);
     const __amdgpu_buffer_rsrc_t rr = rsrc(R, r_bytes);
     const unsigned rowb = (unsigned)G.nrad * 4u;
     const unsigned planeb = rowb * (unsigned)G.nzs;
     const unsigned voff = (unsigned)r * 4u;
-    const uint2* zt = G.ztab + (size_t)s * nl;
+    float acc0[kSG], acc1[kSG];
+#pragma unroll
+    for (int i = 0; i < kSG; i++) acc0[i] = acc1[i] = 0.f;
     unsigned soff = (unsigned)a * (unsigned)nl * planeb;
-    float acc = 0.f;
     for (int li = 0; li < nl; li++, soff += planeb) {
-      const uint2 e = zt[li];
-      const float v0 = ld_f32(rr, voff, soff + (e.x & 0xFFFFu) * rowb);
-      const float v1 = ld_f32(rr, voff, soff + (e.x >> 16) * rowb);
-      acc += lerp1(v0, v1, __uint_as_float(e.y));
+      const uint2* zt = G.ztab + (size_t)li * G.nsinogs + s0;
+      uint2 e[kSG];
+#pragma unroll
+      for (int i = 0; i < kSG; i++) e[i] = zt[i < ns ? i : ns - 1];
+      const unsigned z00 = e[0].x & 0xFFFFu;
+      bool reg = true;
+#pragma unroll
+      for (int i = 0; i < kSG; i++)
+        reg = reg && (e[i].x == ((z00 + i) | ((z00 + i + 1) << 16)));
+      if (reg) {
+        float2 row[kSG + 1];
+#pragma unroll
+        for (int k = 0; k <= kSG; k++) row[k] = ld_f2(rr, voff, soff + (z00 + k) * rowb);
+#pragma unroll
+        for (int i = 0; i < kSG; i++) {
+          const float al = __uint_as_float(e[i].y);
+          acc0[i] += lerp1(row[i].x, row[i + 1].x, al);
+          acc1[i] += lerp1(row[i].y, row[i + 1].y, al);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < kSG; i++) {
+          const float al = __uint_as_float(e[i].y);
+          const float2 v0 = ld_f2(rr, voff, soff + (e[i].x & 0xFFFFu) * rowb);
+          const float2 v1 = ld_f2(rr, voff, soff + (e[i].x >> 16) * rowb);
+          acc0[i] += lerp1(v0.x, v1.x, al);
+          acc1[i] += lerp1(v0.y, v1.y, al);
+        }
+      }
     }
-    const size_t bin = ((size_t)s * G.nang + it) * G.nrad + r;
-    if (o.proj) o.proj[bin] = acc;
-    if constexpr (RATIO) {
-      const float yv = o.yscale * load_store<S>(o.y, bin);
-      float bv = o.bg ? o.yscale * load_store<S>(o.bg, bin) : 0.f;
-      if (bv < 0.f) bv = 0.f;
-      const float den = acc + bv;
-      const float c = (den >= 0.01f) ? yv / den : -1000.f;
-      if (o.corr) o.corr[bin] = c;
-      if (o.corr2) store_packed(G, o.corr2, ag, s, r, make_float2(c, o.mask[bin] ? 1.f : -1000.f));
-      if constexpr (LLH) {
-        if (den > 0.f) ll = (double)yv * log((double)den) - (double)den;
+#pragma unroll
+    for (int i = 0; i < kSG; i++) {
+      if (i >= ns) break;
+      const int s = s0 + i;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int rr2 = r + h;
+        if (rr2 >= G.nrad) break;
+        const float acc = h ? acc1[i] : acc0[i];
+        const size_t bin = ((size_t)s * G.nang + it) * G.nrad + rr2;
+        if (o.proj) o.proj[bin] = acc;
+        if constexpr (RATIO) {
+          const float yv = o.yscale * load_store<S>(o.y, bin);
+          float bv = o.bg ? o.yscale * load_store<S>(o.bg, bin) : 0.f;
+          if (bv < 0.f) bv = 0.f;
+          const float den = acc + bv;
+          const float c = (den >= 0.01f) ? yv / den : -1000.f;
+          if (o.corr) o.corr[bin] = c;
+          if (o.corr2)
+            store_packed(G, o.corr2, ag, s, rr2, make_float2(c, o.mask[bin] ? 1.f : -1000.f));
+          if constexpr (LLH) {
+            if (den > 0.f) ll += (double)yv * log((double)den) - (double)den;
+          }
+        }
       }
     }
   }
   if constexpr (LLH) {
     __shared__ double red[256];
-    const int t = threadIdx.y * kWave + threadIdx.x;
+    const int t = threadIdx.x;
     red[t] = ll;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -297,7 +342,7 @@ size_t rotate_floats_per_angle(const DevGeom& G) {
 }
 
 int forward_llh_blocks(const DevGeom& G, int subset, int nsub) {
-  return ((G.nrad + kWave - 1) / kWave) * ((G.nsinogs + 3) / 4) * nav_of(G, subset, nsub);
+  return ((G.nrad + 127) / 128) * ((G.nsinogs + 4 * kSG - 1) / (4 * kSG)) * nav_of(G, subset, nsub);
 }
 
 hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r_cap_floats,
@@ -312,7 +357,8 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
   if (chunk < 1) return hipErrorInvalidValue;
   FwdArgs o{out.proj, out.corr, out.corr2, out.mask, out.y, out.bg, out.yscale, out.llh_part};
   const bool ratio = out.corr || out.corr2;
-  const unsigned nrb = (G.nrad + kWave - 1) / kWave, nsb = (G.nsinogs + 3) / 4;
+  const unsigned nrb = (G.nrad + kWave - 1) / kWave;
+  const unsigned zrb = (G.nrad + 127) / 128, zgb = (G.nsinogs + 4 * kSG - 1) / (4 * kSG);
   for (int a0 = 0; a0 < nav; a0 += chunk) {
     const int na = min(chunk, nav - a0);
     const unsigned rbytes = (unsigned)((size_t)na * per * sizeof(float));
@@ -321,18 +367,18 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
                        nl);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const unsigned gz = nrb * nsb * na;
-    const unsigned off = nrb * nsb * a0;
+    const unsigned gz = zrb * zgb * na;
+    const unsigned off = zrb * zgb * a0;
     if (!ratio) {
-      hipLaunchKernelGGL((k_zsum<GFR_STORE_F32, false, false>), dim3(gz), dim3(kWave, 4), 0, st, G,
+      hipLaunchKernelGGL((k_zsum<GFR_STORE_F32, false, false>), dim3(gz), dim3(256), 0, st, G,
                          R, rbytes, o, subset, nsub, a0, nl, off);
     } else {
 #define GFR_ZS(S)                                                                           \
   if (out.llh_part)                                                                         \
-    hipLaunchKernelGGL((k_zsum<S, true, true>), dim3(gz), dim3(kWave, 4), 0, st, G, R, rbytes, \
+    hipLaunchKernelGGL((k_zsum<S, true, true>), dim3(gz), dim3(256), 0, st, G, R, rbytes,     \
                        o, subset, nsub, a0, nl, off);                                       \
   else                                                                                      \
-    hipLaunchKernelGGL((k_zsum<S, true, false>), dim3(gz), dim3(kWave, 4), 0, st, G, R,    \
+    hipLaunchKernelGGL((k_zsum<S, true, false>), dim3(gz), dim3(256), 0, st, G, R,         \
                        rbytes, o, subset, nsub, a0, nl, off);
       if (out.storage == GFR_STORE_F16) {
         GFR_ZS(GFR_STORE_F16)
