@@ -121,6 +121,9 @@ def main():
     p = ga.recon_params(mode=1, niter=niter, nsub=args.nsub, storage=storage)
     r = ctx.recon(g, p, d2, None, norm)
     pb = r.get(ga.OUT_PROJ_BLANK)
+    tune = ctx.tuning(g, args.nsub)
+    log("autotuned tiles: back slices/thread=%d, z-sum sinograms/wave=%d (measured=%s)"
+        % tune[:3])
     log("reconstruction resident on device (%.2f GB)" % (ctx.bytes_in_use() / 1e9))
     for _ in range(args.warmup):
         r.step()
@@ -174,6 +177,8 @@ def main():
                      "fp32_tflops": round(flops, 2),
                      "fp32_frac": round(flops / PEAK_FP32_TFLOPS, 4)},
         "loglik_last_pass": ll,
+        "tiles": {"back_slices_per_thread": tune[0], "zsum_sinograms_per_wave": tune[1],
+                  "autotuned": tune[2]},
     }
     r.close()
     ctx.close()

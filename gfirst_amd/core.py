@@ -112,6 +112,18 @@ class Context:
             return None, False
         return self.to_device(x, dtype), True
 
+    # -- tile shapes of the projectors -----------------------------------------
+    def tuning(self, g, nsub):
+        """(back_zb, zsum_sg, measured, ms_back, ms_zsum) for (geometry, nsub)."""
+        zb, sg = C.c_int(), C.c_int()
+        mb, mz = C.c_float(), C.c_float()
+        rc = _check(self.h, lib().gfr_tuning_get(self.h, C.byref(g.c()), nsub, C.byref(zb),
+                                                 C.byref(sg), C.byref(mb), C.byref(mz)))
+        return zb.value, sg.value, bool(rc), mb.value, mz.value
+
+    def set_tuning(self, g, nsub, back_zb, zsum_sg):
+        _check(self.h, lib().gfr_tuning_set(self.h, C.byref(g.c()), nsub, back_zb, zsum_sg))
+
     # -- operators (one per reference wrapper) ---------------------------------
     def fill_cylinder(self, g, val, out=None):
         """CreateInitialCylinder (GFIRST_CPU_Kernels.cpp:35-56)."""

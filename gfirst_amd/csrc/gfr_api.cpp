@@ -25,6 +25,11 @@ int hip_err(gfr_ctx* ctx, hipError_t e, const char* where) {
 
 hipStream_t stream_of(gfr_ctx* ctx, void* s) { return s ? (hipStream_t)s : ctx->stream; }
 
+Tune tune_for(gfr_ctx* ctx, GeomDev* gd, int nsub) {
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return gd->tune[nsub];
+}
+
 void* scratch_acquire(gfr_ctx* ctx, Scratch& s, size_t bytes, hipStream_t st, hipError_t* err) {
   (void)ctx;
   *err = hipSuccess;
@@ -154,7 +159,7 @@ int run_forward(gfr_ctx* ctx, GeomDev* gd, const float* d_img, const FwdOut& o, 
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     R = (float*)scratch_acquire(ctx, ctx->rot, floats * sizeof(float), st, &e);
-    if (R) e = launch_forward(gd->dev, d_img, R, floats, o, subset, nsub, st);
+    if (R) e = launch_forward(gd->dev, d_img, R, floats, o, subset, nsub, gd->tune[nsub], st);
     if (R) scratch_release(ctx->rot, st);
   }
   if (e != hipSuccess) return hip_err(ctx, e, "forward projection");
@@ -376,7 +381,7 @@ int gfr_back_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_corr,
     float2* c2 = (float2*)scratch_acquire(ctx, ctx->pack, bytes, st, &e);
     if (c2) e = launch_pack(gd->dev, d_corr, d_mask, c2, subset, nsub, st);
     if (c2 && e == hipSuccess)
-      e = launch_back_packed(gd->dev, c2, d_numer, d_denom, subset, nsub, st);
+      e = launch_back_packed(gd->dev, c2, d_numer, d_denom, subset, nsub, gd->tune[nsub], st);
     if (c2) scratch_release(ctx->pack, st);
   }
   if (e != hipSuccess) return hip_err(ctx, e, "back projection");
@@ -570,6 +575,38 @@ int gfr_vlor_destroy(gfr_vlor* v) {
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete v;
+  return GFR_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int* back_zb, int* zsum_sg,
+                   float* ms_back, float* ms_zsum) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  if (nsub < 1 || nsub > g->nang) return set_err(ctx, GFR_EINVAL, "nsub out of range");
+  const Tune t = tune_for(ctx, gd, nsub);
+  if (back_zb) *back_zb = t.back_zb;
+  if (zsum_sg) *zsum_sg = t.zsum_sg;
+  if (ms_back) *ms_back = t.ms_back;
+  if (ms_zsum) *ms_zsum = t.ms_zsum;
+  return t.tuned ? 1 : 0;
+}
+
+int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int back_zb, int zsum_sg) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  if (nsub < 1 || nsub > g->nang) return set_err(ctx, GFR_EINVAL, "nsub out of range");
+  auto ok = [](int v) { return v == 4 || v == 8 || v == 16; };
+  if (!ok(back_zb) || !ok(zsum_sg)) return set_err(ctx, GFR_EINVAL, "tile sizes must be 4, 8 or 16");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  Tune& t = gd->tune[nsub];
+  t.back_zb = back_zb;
+  t.zsum_sg = zsum_sg;
+  t.tuned = true;
+  t.ms_back = t.ms_zsum = 0.f;
   return GFR_OK;
 }
 

@@ -41,6 +41,15 @@ struct DevGeom {
 };
 
 struct Taps9 { float g[9]; };
+
+// Tile shapes of the projectors (chosen per geometry and subset count by the
+// autotuner; none of them changes any summation order).
+struct Tune {
+  int back_zb = 8;   // slices per backprojector thread: 4, 8 or 16
+  int zsum_sg = 8;   // consecutive sinograms per z-sum wave: 4, 8 or 16
+  bool tuned = false;
+  float ms_back = 0.f, ms_zsum = 0.f;
+};
 struct Taps5 { float g[5]; };
 
 // Host-side geometry derivation (shared by the API and the kernels' launchers).
@@ -63,6 +72,7 @@ struct GeomDev {
   GeomHost host;
   DevGeom dev;
   void* block = nullptr;  // single device allocation holding the tables
+  std::map<int, Tune> tune;  // per subset count
 };
 
 }  // namespace gfr
@@ -114,6 +124,9 @@ int hip_err(gfr_ctx* ctx, hipError_t e, const char* where);
 
 // Resolves (and caches) the device tables of a geometry.
 GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g);
+// Tile shapes for (geometry, nsub): measured once per context unless
+// GFR_AUTOTUNE=0 (then the defaults); synchronous on first use.
+Tune tune_for(gfr_ctx* ctx, GeomDev* gd, int nsub);
 hipStream_t stream_of(gfr_ctx* ctx, void* s);
 
 // Kernel launchers (gfr_kernels.hip). All return hipError_t of the launch.
@@ -136,11 +149,12 @@ struct FwdOut {
 // Scratch of the rotation stage: floats per angle; the launcher processes the
 // subset's angles in chunks of r_cap_floats / rotate_floats_per_angle().
 size_t rotate_floats_per_angle(const DevGeom& G);
-int forward_llh_blocks(const DevGeom& G, int subset, int nsub);
+int forward_llh_blocks(const DevGeom& G, int subset, int nsub, int sg);
 hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r_cap_floats,
-                          const FwdOut& out, int subset, int nsub, hipStream_t st);
+                          const FwdOut& out, int subset, int nsub, const Tune& tune,
+                          hipStream_t st);
 hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* numer, float* denom,
-                              int subset, int nsub, hipStream_t st);
+                              int subset, int nsub, const Tune& tune, hipStream_t st);
 hipError_t launch_ratio(const DevGeom& G, const float* proj, const float* y, const void* bg,
                         int storage, float yscale, float* corr, float2* corr2,
                         const uint8_t* mask, int subset, int nsub, hipStream_t st);

@@ -14,10 +14,10 @@
 // two coalesced buffer loads with scalar offsets and one lerp.
 //
 // Backprojection (BACKWPROJ_K, GFIRST_GPU_Kernels.cu:109-164): one thread
-// per voxel column and kZB consecutive slices, views in the reference's order
+// per voxel column and ZB consecutive slices, views in the reference's order
 // (segment-major, angle-minor, GFIRST.cu:439-465). Per view the radial
 // interpolation (r0, r1, alpha_u) is shared by the thread's slices; the
-// kZB+1 planes its slices need are fetched together (one memory round trip
+// ZB+1 planes its slices need are fetched together (one memory round trip
 // per view, row addresses without tests thanks to the padded packed rows)
 // and radially interpolated once each. Each slice keeps its own exactly
 // computed plane index and weight, as the reference's per-voxel sampling.
@@ -90,67 +90,65 @@ struct FwdArgs {
 };
 
 // ---------------------------------------------------------------------------
-// k_zsum: one wave per (angle a, kSG consecutive sinograms, 128 radial bins);
+// k_zsum: one wave per (angle a, SG consecutive sinograms, 128 radial bins);
 // lane l owns bins 2l and 2l+1 of each sinogram of the group, so a row load is
 // one contiguous 512-byte dwordx2 access. The z interpolation of a sample is
 // wave-uniform (scalar loads of the transposed z table). When the group's
-// samples at l blend planes z0, z0+1, ..., z0+kSG (consecutive planes: the
-// usual case inside a segment), the kSG+1 rows are loaded once for the group;
+// samples at l blend planes z0, z0+1, ..., z0+SG (consecutive planes: the
+// usual case inside a segment), the SG+1 rows are loaded once for the group;
 // otherwise each LOR loads its own two rows. Every LOR still sums its samples
 // in l order with its own weights. Epilogue: ratio (GFIRST.cu:415-430) and the
 // FP64 Poisson log-likelihood partial of the block.
 // ---------------------------------------------------------------------------
-constexpr int kSG = 8;
-
-template <int S, bool RATIO, bool LLH>
+template <int S, bool RATIO, bool LLH, int SG>
 __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict__ R,
                                               unsigned r_bytes, FwdArgs o, int subset, int nsub,
                                               int a0, int nl, unsigned llh_off) {
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
-  const unsigned nrb = (G.nrad + 127) / 128, ngb = (G.nsinogs + 4 * kSG - 1) / (4 * kSG);
+  const unsigned nrb = (G.nrad + 127) / 128, ngb = (G.nsinogs + 4 * SG - 1) / (4 * SG);
   const unsigned rb = b % nrb;
   const unsigned q = b / nrb;
   const unsigned gb = q % ngb;
   const int a = (int)(q / ngb);
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s0 = __builtin_amdgcn_readfirstlane((int)((gb * 4 + wv) * kSG));
+  const int s0 = __builtin_amdgcn_readfirstlane((int)((gb * 4 + wv) * SG));
   const int r = (int)rb * 128 + 2 * lane;
   const int ag = a0 + a;
   const int it = subset + ag * nsub;
   double ll = 0.0;
   if (s0 < G.nsinogs) {
-    const int ns = min(kSG, G.nsinogs - s0);
+    const int ns = min(SG, G.nsinogs - s0);
     const __amdgpu_buffer_rsrc_t rr = rsrc(R, r_bytes);
     const unsigned rowb = (unsigned)G.nrad * 4u;
     const unsigned planeb = rowb * (unsigned)G.nzs;
     const unsigned voff = (unsigned)r * 4u;
-    float acc0[kSG], acc1[kSG];
+    float acc0[SG], acc1[SG];
 #pragma unroll
-    for (int i = 0; i < kSG; i++) acc0[i] = acc1[i] = 0.f;
+    for (int i = 0; i < SG; i++) acc0[i] = acc1[i] = 0.f;
     unsigned soff = (unsigned)a * (unsigned)nl * planeb;
     for (int li = 0; li < nl; li++, soff += planeb) {
       const uint2* zt = G.ztab + (size_t)li * G.nsinogs + s0;
-      uint2 e[kSG];
+      uint2 e[SG];
 #pragma unroll
-      for (int i = 0; i < kSG; i++) e[i] = zt[i < ns ? i : ns - 1];
+      for (int i = 0; i < SG; i++) e[i] = zt[i < ns ? i : ns - 1];
       const unsigned z00 = e[0].x & 0xFFFFu;
       bool reg = true;
 #pragma unroll
-      for (int i = 0; i < kSG; i++)
+      for (int i = 0; i < SG; i++)
         reg = reg && (e[i].x == ((z00 + i) | ((z00 + i + 1) << 16)));
       if (reg) {
-        float2 row[kSG + 1];
+        float2 row[SG + 1];
 #pragma unroll
-        for (int k = 0; k <= kSG; k++) row[k] = ld_f2(rr, voff, soff + (z00 + k) * rowb);
+        for (int k = 0; k <= SG; k++) row[k] = ld_f2(rr, voff, soff + (z00 + k) * rowb);
 #pragma unroll
-        for (int i = 0; i < kSG; i++) {
+        for (int i = 0; i < SG; i++) {
           const float al = __uint_as_float(e[i].y);
           acc0[i] += lerp1(row[i].x, row[i + 1].x, al);
           acc1[i] += lerp1(row[i].y, row[i + 1].y, al);
         }
       } else {
 #pragma unroll
-        for (int i = 0; i < kSG; i++) {
+        for (int i = 0; i < SG; i++) {
           const float al = __uint_as_float(e[i].y);
           const float2 v0 = ld_f2(rr, voff, soff + (e[i].x & 0xFFFFu) * rowb);
           const float2 v1 = ld_f2(rr, voff, soff + (e[i].x >> 16) * rowb);
@@ -160,7 +158,7 @@ __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict
       }
     }
 #pragma unroll
-    for (int i = 0; i < kSG; i++) {
+    for (int i = 0; i < SG; i++) {
       if (i >= ns) break;
       const int s = s0 + i;
 #pragma unroll
@@ -210,13 +208,13 @@ __global__ void k_pack(DevGeom G, const float* __restrict__ corr, const uint8_t*
   store_packed(G, c2, a, s, r, make_float2(corr[bin], mask[bin] ? 1.f : -1000.f));
 }
 
-constexpr int kZB = 8;  // slices per thread
 
 // Block: 4 waves, each an 8x8 tile of voxel columns (16x16 per block) times
-// kZB slices. Across an 8x8 tile the z tilt of a view moves by a few planes
+// ZB slices. Across an 8x8 tile the z tilt of a view moves by a few planes
 // only, so a wave with no slice inside the view's plane range skips the view
 // (wave-uniform test); the tile's radial bins are a few neighbouring bins, so
 // row loads stay within a few cache lines.
+template <int ZB>
 __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restrict__ c2,
                                               unsigned c2_bytes, float* __restrict__ numer,
                                               float* __restrict__ denom, int subset, int nsub,
@@ -224,16 +222,16 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ix = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
   const int iy = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
-  const int zb = blockIdx.z * kZB;
+  const int zb = blockIdx.z * ZB;
   const bool act = ix < G.res && iy < G.res;
   const __amdgpu_buffer_rsrc_t rc2 = rsrc(c2, c2_bytes);
   const float xv = (float)ix - G.RESM, yv = (float)iy - G.RESM;
   const unsigned rowb = (unsigned)G.nrad * 8u;
   const unsigned nr = (unsigned)(G.nsinogs + 2 * G.nseg);
-  const float fz0 = (float)zb, fzl = (float)(zb + kZB - 1);
-  float n[kZB], d[kZB];
+  const float fz0 = (float)zb, fzl = (float)(zb + ZB - 1);
+  float n[ZB], d[ZB];
 #pragma unroll
-  for (int j = 0; j < kZB; j++) n[j] = d[j] = 0.f;
+  for (int j = 0; j < ZB; j++) n[j] = d[j] = 0.f;
   for (int si = 0; si < G.nseg; si++) {
     const SegEntry se = G.seg[si];
     const int zm = se.zmin;
@@ -257,12 +255,12 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
       axis(u, G.nrad, r0, r1, au);
       const unsigned o0 = (unsigned)r0 * 8u, o1 = (unsigned)r1 * 8u;
       const unsigned vrow = (unsigned)a * nr + segrow;
-      float al[kZB];
-      int ii[kZB];
-      bool inr[kZB];
+      float al[ZB];
+      int ii[ZB];
+      bool inr[ZB];
 #pragma unroll
-      for (int j = 0; j < kZB; j++) {
-        const float w = (j == 0) ? w0 : (j == kZB - 1) ? wl : fmaf(yr, se.fzb, (float)(zb + j)) + 0.5f;
+      for (int j = 0; j < ZB; j++) {
+        const float w = (j == 0) ? w0 : (j == ZB - 1) ? wl : fmaf(yr, se.fzb, (float)(zb + j)) + 0.5f;
         inr[j] = act && (zb + j < G.nzs) && w >= fzm && w < fzhi;
         const float t = w - 0.5f;
         const float f = floorf(t);
@@ -271,31 +269,31 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
       }
       bool reg = true;
 #pragma unroll
-      for (int j = 1; j < kZB; j++) reg = reg && (!inr[j] || ii[j] == ii[0] + j);
-      float av[kZB], bv[kZB];
+      for (int j = 1; j < ZB; j++) reg = reg && (!inr[j] || ii[j] == ii[0] + j);
+      float av[ZB], bv[ZB];
       if (reg) {
-        // planes ii0 + k, k = 0..kZB: slice j blends planes j and j+1
+        // planes ii0 + k, k = 0..ZB: slice j blends planes j and j+1
         const unsigned rb0 = (vrow + (unsigned)ii[0]) * rowb;
-        float c[kZB + 1], q[kZB + 1];
-        float2 v0[kZB + 1], v1[kZB + 1];
+        float c[ZB + 1], q[ZB + 1];
+        float2 v0[ZB + 1], v1[ZB + 1];
 #pragma unroll
-        for (int k = 0; k <= kZB; k++) {
+        for (int k = 0; k <= ZB; k++) {
           v0[k] = ld_f2(rc2, rb0 + o0, (unsigned)k * rowb);
           v1[k] = ld_f2(rc2, rb0 + o1, (unsigned)k * rowb);
         }
 #pragma unroll
-        for (int k = 0; k <= kZB; k++) {
+        for (int k = 0; k <= ZB; k++) {
           c[k] = lerp1(v0[k].x, v1[k].x, au);
           q[k] = lerp1(v0[k].y, v1[k].y, au);
         }
 #pragma unroll
-        for (int j = 0; j < kZB; j++) {
+        for (int j = 0; j < ZB; j++) {
           av[j] = inr[j] ? lerp1(c[j], c[j + 1], al[j]) : 0.f;
           bv[j] = inr[j] ? lerp1(q[j], q[j + 1], al[j]) : 0.f;
         }
       } else {
 #pragma unroll
-        for (int j = 0; j < kZB; j++) {
+        for (int j = 0; j < ZB; j++) {
           av[j] = 0.f;
           bv[j] = 0.f;
           if (inr[j]) {
@@ -308,7 +306,7 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
         }
       }
 #pragma unroll
-      for (int j = 0; j < kZB; j++) {
+      for (int j = 0; j < ZB; j++) {
         float A = av[j], B = bv[j];
         if (B < 0.f || A < 0.f) {  // gaps (GFIRST_GPU_Kernels.cu:157)
           A = 0.f;
@@ -321,7 +319,7 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
   }
   if (!act) return;
 #pragma unroll
-  for (int j = 0; j < kZB; j++) {
+  for (int j = 0; j < ZB; j++) {
     const int iz = zb + j;
     if (iz < G.nzs) {
       const size_t iV = ((size_t)iz * G.res + iy) * G.res + ix;
@@ -341,12 +339,47 @@ size_t rotate_floats_per_angle(const DevGeom& G) {
   return (size_t)(G.l1 - G.l0 + 1) * G.nzs * G.nrad;
 }
 
-int forward_llh_blocks(const DevGeom& G, int subset, int nsub) {
-  return ((G.nrad + 127) / 128) * ((G.nsinogs + 4 * kSG - 1) / (4 * kSG)) * nav_of(G, subset, nsub);
+int forward_llh_blocks(const DevGeom& G, int subset, int nsub, int sg) {
+  return ((G.nrad + 127) / 128) * ((G.nsinogs + 4 * sg - 1) / (4 * sg)) * nav_of(G, subset, nsub);
 }
 
+namespace {
+
+template <int SG>
+hipError_t zsum_launch(const DevGeom& G, const float* R, unsigned rbytes, const FwdArgs& o,
+                       bool ratio, int storage, int subset, int nsub, int a0, int na, int nl,
+                       hipStream_t st) {
+  const unsigned zrb = (G.nrad + 127) / 128, zgb = (G.nsinogs + 4 * SG - 1) / (4 * SG);
+  const unsigned gz = zrb * zgb * na;
+  const unsigned off = zrb * zgb * a0;
+  if (!ratio) {
+    hipLaunchKernelGGL((k_zsum<GFR_STORE_F32, false, false, SG>), dim3(gz), dim3(256), 0, st, G,
+                       R, rbytes, o, subset, nsub, a0, nl, off);
+    return hipGetLastError();
+  }
+#define GFR_ZS(S)                                                                             \
+  if (o.llh_part)                                                                             \
+    hipLaunchKernelGGL((k_zsum<S, true, true, SG>), dim3(gz), dim3(256), 0, st, G, R, rbytes, \
+                       o, subset, nsub, a0, nl, off);                                         \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_zsum<S, true, false, SG>), dim3(gz), dim3(256), 0, st, G, R,       \
+                       rbytes, o, subset, nsub, a0, nl, off);
+  if (storage == GFR_STORE_F16) {
+    GFR_ZS(GFR_STORE_F16)
+  } else if (storage == GFR_STORE_BF16) {
+    GFR_ZS(GFR_STORE_BF16)
+  } else {
+    GFR_ZS(GFR_STORE_F32)
+  }
+#undef GFR_ZS
+  return hipGetLastError();
+}
+
+}  // namespace
+
 hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r_cap_floats,
-                          const FwdOut& out, int subset, int nsub, hipStream_t st) {
+                          const FwdOut& out, int subset, int nsub, const Tune& tune,
+                          hipStream_t st) {
   const int nav = nav_of(G, subset, nsub);
   const int nl = G.l1 - G.l0 + 1;
   const size_t per = rotate_floats_per_angle(G);
@@ -358,7 +391,6 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
   FwdArgs o{out.proj, out.corr, out.corr2, out.mask, out.y, out.bg, out.yscale, out.llh_part};
   const bool ratio = out.corr || out.corr2;
   const unsigned nrb = (G.nrad + kWave - 1) / kWave;
-  const unsigned zrb = (G.nrad + 127) / 128, zgb = (G.nsinogs + 4 * kSG - 1) / (4 * kSG);
   for (int a0 = 0; a0 < nav; a0 += chunk) {
     const int na = min(chunk, nav - a0);
     const unsigned rbytes = (unsigned)((size_t)na * per * sizeof(float));
@@ -367,29 +399,11 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
                        nl);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const unsigned gz = zrb * zgb * na;
-    const unsigned off = zrb * zgb * a0;
-    if (!ratio) {
-      hipLaunchKernelGGL((k_zsum<GFR_STORE_F32, false, false>), dim3(gz), dim3(256), 0, st, G,
-                         R, rbytes, o, subset, nsub, a0, nl, off);
-    } else {
-#define GFR_ZS(S)                                                                           \
-  if (out.llh_part)                                                                         \
-    hipLaunchKernelGGL((k_zsum<S, true, true>), dim3(gz), dim3(256), 0, st, G, R, rbytes,     \
-                       o, subset, nsub, a0, nl, off);                                       \
-  else                                                                                      \
-    hipLaunchKernelGGL((k_zsum<S, true, false>), dim3(gz), dim3(256), 0, st, G, R,         \
-                       rbytes, o, subset, nsub, a0, nl, off);
-      if (out.storage == GFR_STORE_F16) {
-        GFR_ZS(GFR_STORE_F16)
-      } else if (out.storage == GFR_STORE_BF16) {
-        GFR_ZS(GFR_STORE_BF16)
-      } else {
-        GFR_ZS(GFR_STORE_F32)
-      }
-#undef GFR_ZS
+    switch (tune.zsum_sg) {
+      case 4: e = zsum_launch<4>(G, R, rbytes, o, ratio, out.storage, subset, nsub, a0, na, nl, st); break;
+      case 16: e = zsum_launch<16>(G, R, rbytes, o, ratio, out.storage, subset, nsub, a0, na, nl, st); break;
+      default: e = zsum_launch<8>(G, R, rbytes, o, ratio, out.storage, subset, nsub, a0, na, nl, st); break;
     }
-    e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -408,13 +422,17 @@ hipError_t launch_pack(const DevGeom& G, const float* corr, const uint8_t* mask,
 }
 
 hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* numer, float* denom,
-                              int subset, int nsub, hipStream_t st) {
+                              int subset, int nsub, const Tune& tune, hipStream_t st) {
   const int nav = nav_of(G, subset, nsub);
   const size_t bytes = packed_bytes(G, subset, nsub);
   if (bytes >= 0xFFFFFFF0u) return hipErrorInvalidValue;
-  dim3 grid((G.res + 15) / 16, (G.res + 15) / 16, (G.nzs + kZB - 1) / kZB);
-  hipLaunchKernelGGL(k_back, grid, dim3(256), 0, st, G, corr2, (unsigned)bytes, numer, denom,
-                     subset, nsub, nav);
+#define GFR_BK(ZB)                                                                                hipLaunchKernelGGL(k_back<ZB>, dim3((G.res + 15) / 16, (G.res + 15) / 16, (G.nzs + ZB - 1) / ZB),                      dim3(256), 0, st, G, corr2, (unsigned)bytes, numer, denom, subset, nsub, nav)
+  switch (tune.back_zb) {
+    case 4: GFR_BK(4); break;
+    case 16: GFR_BK(16); break;
+    default: GFR_BK(8); break;
+  }
+#undef GFR_BK
   return hipGetLastError();
 }
 

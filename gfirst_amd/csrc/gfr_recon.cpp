@@ -6,6 +6,7 @@
 // Loop nest: iter -> type (doubles, then triples) -> subset. Subset s holds
 // the angles theta with theta % S == s; S = 1 is the reference's loop.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -39,6 +40,7 @@ struct gfr_recon {
   int llh_valid = 0;
   std::vector<void*> owned;
   size_t bytes = 0;
+  Tune tune;
   int iter = 1, type = 0, subset = 0;
   long long passes = 0;
   bool done = false;
@@ -128,6 +130,87 @@ struct PhaseMark {
   }
   ~PhaseMark() { end(); }
 };
+
+// Times each tile shape of the two projectors on this session's data (subset
+// 0, doubles) and keeps the fastest; the result is shared with later sessions
+// of the same geometry and subset count. Any shape gives identical results.
+hipError_t autotune(gfr_recon* r) {
+  GeomDev* gd = r->gd;
+  const int S = r->p.nsub;
+  {
+    std::lock_guard<std::mutex> lk(r->ctx->mu);
+    const Tune& c = gd->tune[S];
+    if (c.tuned) {
+      r->tune = c;
+      return hipSuccess;
+    }
+  }
+  Tune t;
+  const char* env = getenv("GFR_AUTOTUNE");
+  if (env && env[0] == '0') {
+    r->tune = t;
+    return hipSuccess;
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return hipErrorUnknown;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return hipErrorUnknown;
+  }
+  const DevGeom& G = gd->dev;
+  FwdOut fo;
+  fo.corr2 = r->corr2;
+  fo.mask = r->mask;
+  fo.y = r->yD;
+  fo.bg = r->bD;
+  fo.storage = r->p.storage;
+  fo.yscale = (float)r->g.res;
+  fo.llh_part = r->llh_part;
+  hipError_t e = hipSuccess;
+  auto time_it = [&](auto&& launch) -> float {
+    float best = 1e30f;
+    for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+      e = hipEventRecord(e0, r->st);
+      if (e == hipSuccess) e = launch();
+      if (e == hipSuccess) e = hipEventRecord(e1, r->st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && rep > 0)
+        best = ms < best ? ms : best;
+    }
+    return best;
+  };
+  const int cand[3] = {4, 8, 16};
+  float bz = 1e30f, bb = 1e30f;
+  for (int c : cand) {
+    Tune tt = t;
+    tt.zsum_sg = c;
+    const float ms = time_it([&] { return launch_forward(G, r->xb, r->R, r->r_floats, fo, 0, S, tt, r->st); });
+    if (ms < bz) {
+      bz = ms;
+      t.zsum_sg = c;
+    }
+  }
+  for (int c : cand) {
+    Tune tt = t;
+    tt.back_zb = c;
+    const float ms = time_it([&] { return launch_back_packed(G, r->corr2, r->num, r->den, 0, S, tt, r->st); });
+    if (ms < bb) {
+      bb = ms;
+      t.back_zb = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return e;
+  t.tuned = true;
+  t.ms_zsum = bz;
+  t.ms_back = bb;
+  r->tune = t;
+  std::lock_guard<std::mutex> lk(r->ctx->mu);
+  gd->tune[S] = t;
+  return hipSuccess;
+}
 
 int rfail(gfr_recon* r, int code) {
   gfr_recon_destroy(r);
@@ -220,7 +303,7 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   e = launch_fill_cylinder(G, 1.f, r->xb, st);
   FwdOut fo;
   fo.proj = r->pb;
-  if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fo, 0, 1, st);
+  if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fo, 0, 1, r->tune, st);
   if (e == hipSuccess) e = hipMemcpyAsync(stage, norm, pl.sino_f32, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = launch_calib_mask(G, r->pb, stage, r->mask, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -252,10 +335,11 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
     if (e == hipSuccess) e = launch_blur_xy(G, t9, r->xT, r->xb, st);
     FwdOut fc;
     fc.proj = r->ptc;
-    if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fc, 0, 1, st);
+    if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fc, 0, 1, r->tune, st);
     if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create vlor cache"));
   }
   e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = autotune(r);
   if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create sync"));
   r->done = (p->niter == 0);
   *out = r;
@@ -288,7 +372,7 @@ int gfr_recon_step(gfr_recon* r) {
       PhaseMark pm(r, GFR_PHASE_FORWARD);
       FwdOut fc;
       fc.proj = r->ptc;
-      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fc, s, S, st));
+      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fc, s, S, r->tune, st));
     }
     PhaseMark pm(r, GFR_PHASE_VLOR);
     int rc = gfr_vlor_assign(ctx, r->vl, r->ptc, r->dv, s, S, st);
@@ -309,15 +393,16 @@ int gfr_recon_step(gfr_recon* r) {
       fo.storage = p.storage;
       fo.yscale = (float)r->g.res;
       fo.llh_part = r->llh_part;
-      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fo, s, S, st));
+      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fo, s, S, r->tune, st));
     }
-    GFR_HIP(ctx, launch_llh_final(r->llh_part, forward_llh_blocks(G, s, S), r->d_llh, st));
+    GFR_HIP(ctx, launch_llh_final(r->llh_part, forward_llh_blocks(G, s, S, r->tune.zsum_sg),
+                                  r->d_llh, st));
     r->llh_valid = 1;
   }
   // Backprojection of the subset (GFIRST.cu:436-469) and update (:482-502).
   {
     PhaseMark pm(r, GFR_PHASE_BACK);
-    GFR_HIP(ctx, launch_back_packed(G, r->corr2, r->num, r->den, s, S, st));
+    GFR_HIP(ctx, launch_back_packed(G, r->corr2, r->num, r->den, s, S, r->tune, st));
   }
   {
     PhaseMark pm(r, GFR_PHASE_UPDATE);

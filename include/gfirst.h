@@ -123,6 +123,17 @@ int gfr_memcpy_d2h(gfr_ctx* ctx, void* dst, const void* src, size_t bytes, void*
 /* Zero fill with a kernel (CleanMemory, GPU:207-219). */
 int gfr_zero(gfr_ctx* ctx, void* dptr, size_t bytes, void* stream);
 
+/* Tile shapes of the projectors for (geometry, nsub): back_zb = slices per
+ * backprojector thread (4, 8, 16), zsum_sg = sinograms per forward z-sum wave
+ * (4, 8, 16). A reconstruction session measures them once per context (env
+ * GFR_AUTOTUNE=0 keeps the defaults 8/8); per-operation calls use the
+ * context's current choice. Results do not depend on the shapes. No
+ * counterpart in the reference (its launch shapes are fixed, GPU:105-107,
+ * 166-171). get returns 1 if the shapes were measured, 0 if defaults. */
+int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int* back_zb, int* zsum_sg,
+                   float* ms_back, float* ms_zsum);
+int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int back_zb, int zsum_sg);
+
 /* ---------------- per-operation entry points (stream ordered) ---------------- */
 
 /* CreateInitialCylinder (CPU:35-56) into a device image. */

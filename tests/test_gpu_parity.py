@@ -130,6 +130,26 @@ def test_back_project_c1_subset_vs_oracle(gctx, orc):
     close(d, de, rtol=1e-6, atol=1e-5, what="denom c1")
 
 
+@pytest.mark.parametrize("zb,sg", [(4, 4), (8, 16), (16, 8)])
+def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg):
+    """Every autotuner candidate gives the oracle's bits (forward and back)."""
+    g = C0
+    gctx.set_tuning(g, 4, zb, sg)
+    assert gctx.tuning(g, 4)[:3] == (zb, sg, True)
+    x = rand_img(g, 30)
+    close(gctx.forward_project(g, x, 2, 4), orc.forward(g, x, 2, 4), what="fwd sg=%d" % sg)
+    rng = np.random.default_rng(31)
+    nd = g.ndata
+    corr = rng.uniform(0, 3, nd).astype(np.float32)
+    corr[rng.uniform(size=nd) < 0.05] = -1000.0
+    valid = rng.uniform(size=nd) < 0.93
+    n, d = gctx.back_project(g, corr, valid.astype(np.uint8), 2, 4)
+    ne, de = orc.back(g, corr, np.where(valid, 1, -1000).astype(np.float32), 2, 4)
+    close(n, ne, rtol=0, what="numer zb=%d" % zb)
+    close(d, de, rtol=0, what="denom zb=%d" % zb)
+    gctx.set_tuning(g, 4, 8, 8)
+
+
 @pytest.mark.parametrize("g", [SMALL, C0], ids=["small", "c0"])
 def test_update(gctx, orc, g):
     rng = np.random.default_rng(9)
