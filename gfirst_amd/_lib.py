@@ -90,6 +90,8 @@ SIGNATURES = {
     "gfr_vlor_destroy": (_I, [_V]),
     "gfr_read_sinogram_file": (_I, [C.c_char_p, _LL, _V, C.POINTER(_I), C.POINTER(_I)]),
     "gfr_read_vlor_file": (_LL, [C.c_char_p, _V, _LL]),
+    "gfr_lor_bin": (_LL, [_G, _V, _V]),
+    "gfr_vlor_bin_hits": (_LL, [_G, _V, _LL, _F, _V]),
     "gfr_recon_create": (_I, [_V, _G, C.POINTER(GfrReconParams), _V, _V, _V, _V, _LL,
                               C.POINTER(_V)]),
     "gfr_recon_footprint": (_LL, [_G, C.POINTER(GfrReconParams)]),

@@ -83,7 +83,7 @@ GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g) {
   if (it != ctx->geoms.end()) return it->second;
   GeomDev* gd = new GeomDev();
   std::string err;
-  if (derive_geometry(g, &gd->host, &err) != GFR_OK) {
+  if (derive_geometry(g, &gd->host, &err, true) != GFR_OK) {
     set_err(ctx, GFR_EINVAL, err);
     delete gd;
     return nullptr;

@@ -14,7 +14,7 @@ namespace {
 constexpr float kPI = 3.1415926f;
 }  // namespace
 
-int derive_geometry(const gfr_geometry* g, GeomHost* out, std::string* err) {
+int derive_geometry(const gfr_geometry* g, GeomHost* out, std::string* err, bool with_ztab) {
   auto fail = [&](const char* m) {
     if (err) *err = m;
     return GFR_EINVAL;
@@ -87,6 +87,7 @@ int derive_geometry(const gfr_geometry* g, GeomHost* out, std::string* err) {
   // (GFIRST_GPU_Kernels.cu:88, 96), through the clamp-to-edge linear filter.
   // Same float operations as the device code (fmaf is exactly rounded on both).
   if (g->nzs > 0xFFFF) return fail("geometry: nzs too large");
+  if (!with_ztab) return GFR_OK;
   const int nl = h.info.lambda1 - h.info.lambda0 + 1;
   h.ztab.resize((size_t)acc * nl);
   for (int s = 0; s < acc; s++) {

@@ -64,7 +64,8 @@ struct GeomHost {
   float RADM, RESM, fxf, fxb;
 };
 
-int derive_geometry(const gfr_geometry* g, GeomHost* out, std::string* err);
+// with_ztab: also build the per-sample z table (device tables only need it).
+int derive_geometry(const gfr_geometry* g, GeomHost* out, std::string* err, bool with_ztab = false);
 void psf_taps(float psf, float* g9);
 void gz_taps(float* g5);
 

@@ -216,6 +216,24 @@ int gfr_read_sinogram_file(const char* path, long long ndata, float* out, int* i
  * Call with rec=NULL to get the count; returns count or <0. */
 long long gfr_read_vlor_file(const char* path, int32_t* rec, long long cap);
 
+/* ---------------- list-mode binning (host; no counterpart in the reference,
+ * which takes VLORs already binned, DRV:213-236) ---------------- */
+
+/* Bin of the LOR through points a and b (mm, scanner frame, z axial, origin at
+ * the centre) in the sinogram convention of the forward projector (GPU:75-96):
+ * angle index by the LOR direction, radial bin by its distance to the axis in
+ * units of pitch/2, segment by its ring difference (axial slope x DIAM/pitch)
+ * and plane by its axial position at the point closest to the axis. Returns
+ * the 0-based bin or -1 if the LOR lies outside the sinogram. */
+long long gfr_lor_bin(const gfr_geometry* g, const double* a, const double* b);
+/* Triple events as 3 hits (phi, z) on a cylinder of the given radius (6 floats
+ * per event; hit 0 is the shared vertex): rec[2e], rec[2e+1] = 1-based bins of
+ * the LORs (hit0, hit1) and (hit0, hit2), 0 where a LOR is outside the
+ * sinogram (the VLORs.raw record format). Returns the number of events with
+ * both LORs inside. */
+long long gfr_vlor_bin_hits(const gfr_geometry* g, const float* hits, long long nev, float radius,
+                            int32_t* rec);
+
 /* ---------------- whole reconstruction (device-resident session) ---------------- */
 
 /* osem3d (DRV:44-575) as a session: uploads the data once, keeps every
