@@ -7,11 +7,12 @@ GPU, and every operator raises if the native library is missing.
 """
 from ._lib import (GFR_EINVAL, GFR_EHIP, GFR_EIO, GFR_ENOMEM, GFR_ESTATE, GFR_OK, OUT_PROJ_BLANK,
                    OUT_VLOR, OUT_XD, OUT_XT, STORE_BF16, STORE_F16, STORE_F32, GfrError, lib)
-from .core import Context, DeviceArray, Recon, Vlor, footprint, recon_params
+from .core import (Batch, Context, DeviceArray, Recon, Vlor, batch_footprint, batch_params,
+                   footprint, recon_params)
 from .geometry import C0, C1, PRESETS, REF, SMALL, TINY, Geometry, nseg_for
 
 __all__ = [
-    "Context", "DeviceArray", "Recon", "Vlor", "Geometry", "recon_params", "footprint", "lib",
+    "Context", "DeviceArray", "Recon", "Vlor", "Batch", "batch_params", "batch_footprint", "Geometry", "recon_params", "footprint", "lib",
     "GfrError", "REF", "C0", "C1", "TINY", "SMALL", "PRESETS", "nseg_for", "STORE_F32",
     "STORE_F16", "STORE_BF16", "OUT_XD", "OUT_XT", "OUT_PROJ_BLANK", "OUT_VLOR", "GFR_OK",
     "GFR_EINVAL", "GFR_EHIP", "GFR_ENOMEM", "GFR_EIO", "GFR_ESTATE",

@@ -39,6 +39,11 @@ class GfrReconParams(C.Structure):
                 ("storage", C.c_int)]
 
 
+class GfrBatchParams(C.Structure):
+    _fields_ = [("storage", C.c_int), ("max_frames", C.c_int), ("min_nsub", C.c_int),
+                ("lanes", C.c_int), ("psf_fw", C.c_float), ("psf_bw", C.c_float)]
+
+
 class GfrError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("gfr error %d: %s" % (code, msg))
@@ -105,6 +110,21 @@ SIGNATURES = {
     "gfr_recon_destroy": (_I, [_V]),
     "gfr_recon_profile": (_I, [_V, _I]),
     "gfr_recon_phase_times": (_I, [_V, _V, _V]),
+    "gfr_batch_footprint": (_LL, [_G, C.POINTER(GfrBatchParams)]),
+    "gfr_batch_frame_footprint": (_LL, [_G, C.POINTER(GfrBatchParams), _I]),
+    "gfr_batch_create": (_I, [_V, _G, C.POINTER(GfrBatchParams), _V, C.POINTER(_V)]),
+    "gfr_batch_add": (_I, [_V, _I, _I, _V, _V, C.POINTER(_I)]),
+    "gfr_batch_step": (_I, [_V]),
+    "gfr_batch_run": (_I, [_V]),
+    "gfr_batch_cursor": (_I, [_V, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_LL)]),
+    "gfr_batch_get": (_I, [_V, _I, _V]),
+    "gfr_batch_loglik": (_I, [_V, _I, C.POINTER(C.c_double)]),
+    "gfr_batch_remove": (_I, [_V, _I]),
+    "gfr_batch_use_graph": (_I, [_V, _I]),
+    "gfr_batch_stats": (_I, [_V, C.POINTER(_LL), C.POINTER(_LL)]),
+    "gfr_batch_stream": (_V, [_V]),
+    "gfr_batch_destroy": (_I, [_V]),
+    "gfr_poisson": (_I, [_V, _V, C.c_double, C.c_ulonglong, _V, _LL, _V]),
 }
 
 PHASES = ("blur", "forward", "vlor", "back", "update", "jbf")

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import GfrError, GfrReconParams, lib
+from ._lib import GfrBatchParams, GfrError, GfrReconParams, lib
 
 
 def _check(ctx, rc):
@@ -227,6 +227,20 @@ class Context:
     def recon(self, g, params, d2, t2, norm, records=None):
         return Recon(self, g, params, d2, t2, norm, records)
 
+    def batch(self, g, params, norm):
+        return Batch(self, g, params, norm)
+
+    def poisson(self, mean, scale, seed, out=None):
+        """Seeded Poisson(scale * mean) draws on the device (float counts)."""
+        m, _ = self._dev(mean)
+        o = out if out is not None else self.array(m.n)
+        _check(self.h, lib().gfr_poisson(self.h, m.ptr, float(scale), int(seed), o.ptr, m.n,
+                                         None))
+        if out is not None:
+            return o
+        self.sync()
+        return o.download()
+
 
 class Vlor:
     """VLOR soft-assignment state (GFIRST.cu:229-247, 373-388)."""
@@ -339,6 +353,105 @@ class Recon:
     def close(self):
         if self.h:
             lib().gfr_recon_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_params(storage=_lib.STORE_F32, max_frames=64, min_nsub=8, lanes=2, psf_fw=1.5,
+                 psf_bw=1.5):
+    return GfrBatchParams(storage, max_frames, min_nsub, lanes, psf_fw, psf_bw)
+
+
+def batch_footprint(g, params, frames=0, has_bg=False):
+    """Device bytes of a batch holding `frames` frames."""
+    base = lib().gfr_batch_footprint(C.byref(g.c()), C.byref(params))
+    per = lib().gfr_batch_frame_footprint(C.byref(g.c()), C.byref(params), 1 if has_bg else 0)
+    return base + frames * per
+
+
+class Batch:
+    """Frames of one geometry advanced one subset per step, each step a
+    replay of one captured hipGraph (doubles OSEM, GFIRST.cu:44-575 mode 1)."""
+
+    def __init__(self, ctx, g, params, norm):
+        self.ctx, self.g, self.params = ctx, g, params
+        norm = np.ascontiguousarray(norm, np.float32).reshape(-1)
+        if norm.size < g.ndata:
+            raise ValueError("norm must hold NDATA floats")
+        h = C.c_void_p()
+        _check(ctx.h, lib().gfr_batch_create(ctx.h, C.byref(g.c()), C.byref(params),
+                                             norm.ctypes.data, C.byref(h)))
+        self.h = h.value
+
+    def add(self, niter, nsub, prompts, bg=None):
+        """Adds a frame from NDATA prompts (numpy or DeviceArray) and an
+        optional background; returns its id."""
+        keep = []
+
+        def ptr(a):
+            if a is None:
+                return None
+            if isinstance(a, DeviceArray):
+                return a.ptr
+            a = np.ascontiguousarray(a, np.float32).reshape(-1)
+            if a.size != self.g.ndata:
+                raise ValueError("frame sinograms must hold NDATA floats")
+            keep.append(a)
+            return a.ctypes.data
+
+        f = C.c_int()
+        _check(self.ctx.h, lib().gfr_batch_add(self.h, niter, nsub, ptr(prompts), ptr(bg),
+                                               C.byref(f)))
+        return f.value
+
+    def step(self):
+        return _check(self.ctx.h, lib().gfr_batch_step(self.h))
+
+    def run(self):
+        _check(self.ctx.h, lib().gfr_batch_run(self.h))
+
+    def cursor(self, frame):
+        it, s, n = C.c_int(), C.c_int(), C.c_longlong()
+        done = _check(self.ctx.h, lib().gfr_batch_cursor(self.h, frame, C.byref(it), C.byref(s),
+                                                         C.byref(n)))
+        return dict(iter=it.value, subset=s.value, passes=n.value, done=bool(done))
+
+    def get(self, frame):
+        out = np.empty(self.g.nvox, np.float32)
+        _check(self.ctx.h, lib().gfr_batch_get(self.h, frame, out.ctypes.data))
+        return out
+
+    def loglik(self, frame):
+        v = C.c_double()
+        _check(self.ctx.h, lib().gfr_batch_loglik(self.h, frame, C.byref(v)))
+        return v.value
+
+    def remove(self, frame):
+        _check(self.ctx.h, lib().gfr_batch_remove(self.h, frame))
+
+    def use_graph(self, enable=True):
+        _check(self.ctx.h, lib().gfr_batch_use_graph(self.h, 1 if enable else 0))
+
+    def stats(self):
+        a, b = C.c_longlong(), C.c_longlong()
+        _check(self.ctx.h, lib().gfr_batch_stats(self.h, C.byref(a), C.byref(b)))
+        return dict(launches=a.value, builds=b.value)
+
+    @property
+    def stream(self):
+        return lib().gfr_batch_stream(self.h)
+
+    def sync(self):
+        self.ctx.sync(self.stream)
+
+    def close(self):
+        if self.h:
+            lib().gfr_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -159,7 +159,7 @@ int run_forward(gfr_ctx* ctx, GeomDev* gd, const float* d_img, const FwdOut& o, 
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     R = (float*)scratch_acquire(ctx, ctx->rot, floats * sizeof(float), st, &e);
-    if (R) e = launch_forward(gd->dev, d_img, R, floats, o, subset, nsub, gd->tune[nsub], st);
+    if (R) e = launch_forward(gd->dev, d_img, R, floats, o, subset, nsub, nullptr, gd->tune[nsub], st);
     if (R) scratch_release(ctx->rot, st);
   }
   if (e != hipSuccess) return hip_err(ctx, e, "forward projection");
@@ -381,7 +381,7 @@ int gfr_back_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_corr,
     float2* c2 = (float2*)scratch_acquire(ctx, ctx->pack, bytes, st, &e);
     if (c2) e = launch_pack(gd->dev, d_corr, d_mask, c2, subset, nsub, st);
     if (c2 && e == hipSuccess)
-      e = launch_back_packed(gd->dev, c2, d_numer, d_denom, subset, nsub, gd->tune[nsub], st);
+      e = launch_back_packed(gd->dev, c2, d_numer, d_denom, subset, nsub, nullptr, gd->tune[nsub], st);
     if (c2) scratch_release(ctx->pack, st);
   }
   if (e != hipSuccess) return hip_err(ctx, e, "back projection");

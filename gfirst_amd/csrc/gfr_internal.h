@@ -151,11 +151,18 @@ struct FwdOut {
 // subset's angles in chunks of r_cap_floats / rotate_floats_per_angle().
 size_t rotate_floats_per_angle(const DevGeom& G);
 int forward_llh_blocks(const DevGeom& G, int subset, int nsub, int sg);
+// dsub: optional device pointer to the subset index (read by the kernels, so
+// a captured graph stays valid while the index changes); grids then cover
+// the largest subset.
 hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r_cap_floats,
-                          const FwdOut& out, int subset, int nsub, const Tune& tune,
-                          hipStream_t st);
+                          const FwdOut& out, int subset, int nsub, const int* dsub,
+                          const Tune& tune, hipStream_t st);
 hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* numer, float* denom,
-                              int subset, int nsub, const Tune& tune, hipStream_t st);
+                              int subset, int nsub, const int* dsub, const Tune& tune,
+                              hipStream_t st);
+// FP64 log-likelihood of the last forward pass (sum of its block partials).
+hipError_t launch_llh_sum(const DevGeom& G, const double* part, int subset, int nsub,
+                          const int* dsub, int sg, double* out, hipStream_t st);
 hipError_t launch_ratio(const DevGeom& G, const float* proj, const float* y, const void* bg,
                         int storage, float yscale, float* corr, float2* corr2,
                         const uint8_t* mask, int subset, int nsub, hipStream_t st);
@@ -177,6 +184,20 @@ hipError_t launch_scale_copy(const float* in, float s, float* out, long long n, 
 hipError_t launch_convert_store(const float* in, void* out, int storage, long long n,
                                 hipStream_t st);
 hipError_t launch_llh_final(const double* part, int n, double* out, hipStream_t st);
+// sub[slots[i]] = (sub[slots[i]] + 1) mod nsub[slots[i]] for i < n.
+hipError_t launch_advance(int* sub, const int* nsub, const int* slots, int n, hipStream_t st);
+// Synthetic measurements (gfr_synth.hip).
+hipError_t launch_poisson(const float* mean, double scale, uint64_t seed, float* out, long long n,
+                          hipStream_t st);
+
+// Largest number of FP64 partials a forward pass with `nsub` subsets writes
+// (any tile shape).
+int llh_capacity(const GeomHost& h, int nsub);
+// Times each tile shape of the two projectors on the given buffers (subset 0)
+// and returns the fastest; any shape gives identical results.
+hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float* img, float* R,
+                       size_t r_floats, const float2* corr2, float* num, float* den,
+                       hipStream_t st, Tune* out);
 
 // VLOR.
 hipError_t launch_vlor_blankt(const float* yT, const float* v0, float res, float* blankt,

@@ -345,6 +345,24 @@ hipError_t launch_fill_cylinder(const DevGeom& G, float val, float* img, hipStre
   return hipGetLastError();
 }
 
+namespace {
+// Advances the subset index of each listed slot: sub = (sub + 1) mod nsub.
+__global__ void k_advance(int* __restrict__ sub, const int* __restrict__ nsub,
+                          const int* __restrict__ slots, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = slots[i];
+  const int v = sub[s] + 1;
+  sub[s] = v >= nsub[s] ? 0 : v;
+}
+}  // namespace
+
+hipError_t launch_advance(int* sub, const int* nsub, const int* slots, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_advance, dim3((n + 255) / 256), dim3(256), 0, st, sub, nsub, slots, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t st) {
   if (bytes == 0) return hipSuccess;
   size_t nw = bytes / 4;

@@ -47,7 +47,8 @@ __device__ __forceinline__ float2 ld_f2(__amdgpu_buffer_rsrc_t r, unsigned voff,
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_rotate(DevGeom G, const float* __restrict__ img,
                                                 float* __restrict__ R, int subset, int nsub,
-                                                int a0, int nl) {
+                                                const int* __restrict__ dsub, int a0, int nl) {
+  if (dsub) subset = *dsub;
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const unsigned nrb = (G.nrad + kWave - 1) / kWave, nlb = (nl + 3) / 4;
   const unsigned rb = b % nrb;
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(256) void k_rotate(DevGeom G, const float* __restri
   const int a = (int)(q / G.nzs);
   const int r = rb * kWave + threadIdx.x;
   const int li = lb * 4 + threadIdx.y;
-  if (r >= G.nrad || li >= nl) return;
+  if (r >= G.nrad || li >= nl || a0 + a >= (G.nang - subset + nsub - 1) / nsub) return;
   const int it = subset + (a0 + a) * nsub;
   const float st = G.sin_t[it], ct = G.cos_t[it];
   const float x0 = G.fxf * ((float)r - G.RADM);
@@ -103,7 +104,9 @@ struct FwdArgs {
 template <int S, bool RATIO, bool LLH, int SG>
 __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict__ R,
                                               unsigned r_bytes, FwdArgs o, int subset, int nsub,
-                                              int a0, int nl, unsigned llh_off) {
+                                              const int* __restrict__ dsub, int a0, int nl,
+                                              unsigned llh_off) {
+  if (dsub) subset = *dsub;
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const unsigned nrb = (G.nrad + 127) / 128, ngb = (G.nsinogs + 4 * SG - 1) / (4 * SG);
   const unsigned rb = b % nrb;
@@ -115,6 +118,7 @@ __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict
   const int r = (int)rb * 128 + 2 * lane;
   const int ag = a0 + a;
   const int it = subset + ag * nsub;
+  if (ag >= (G.nang - subset + nsub - 1) / nsub) return;  // whole block: angle not in subset
   double ll = 0.0;
   if (s0 < G.nsinogs) {
     const int ns = min(SG, G.nsinogs - s0);
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict
       if (t < w) red[t] += red[t + w];
       __syncthreads();
     }
-    if (t == 0) o.llh_part[llh_off + blockIdx.x] = red[0];
+    if (t == 0) o.llh_part[llh_off + b] = red[0];
   }
 }
 
@@ -218,7 +222,9 @@ template <int ZB>
 __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restrict__ c2,
                                               unsigned c2_bytes, float* __restrict__ numer,
                                               float* __restrict__ denom, int subset, int nsub,
-                                              int nav) {
+                                              const int* __restrict__ dsub) {
+  if (dsub) subset = *dsub;
+  const int nav = (G.nang - subset + nsub - 1) / nsub;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ix = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
   const int iy = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
@@ -347,23 +353,23 @@ namespace {
 
 template <int SG>
 hipError_t zsum_launch(const DevGeom& G, const float* R, unsigned rbytes, const FwdArgs& o,
-                       bool ratio, int storage, int subset, int nsub, int a0, int na, int nl,
-                       hipStream_t st) {
+                       bool ratio, int storage, int subset, int nsub, const int* dsub, int a0,
+                       int na, int nl, hipStream_t st) {
   const unsigned zrb = (G.nrad + 127) / 128, zgb = (G.nsinogs + 4 * SG - 1) / (4 * SG);
   const unsigned gz = zrb * zgb * na;
   const unsigned off = zrb * zgb * a0;
   if (!ratio) {
     hipLaunchKernelGGL((k_zsum<GFR_STORE_F32, false, false, SG>), dim3(gz), dim3(256), 0, st, G,
-                       R, rbytes, o, subset, nsub, a0, nl, off);
+                       R, rbytes, o, subset, nsub, dsub, a0, nl, off);
     return hipGetLastError();
   }
 #define GFR_ZS(S)                                                                             \
   if (o.llh_part)                                                                             \
     hipLaunchKernelGGL((k_zsum<S, true, true, SG>), dim3(gz), dim3(256), 0, st, G, R, rbytes, \
-                       o, subset, nsub, a0, nl, off);                                         \
+                       o, subset, nsub, dsub, a0, nl, off);                                   \
   else                                                                                        \
     hipLaunchKernelGGL((k_zsum<S, true, false, SG>), dim3(gz), dim3(256), 0, st, G, R,       \
-                       rbytes, o, subset, nsub, a0, nl, off);
+                       rbytes, o, subset, nsub, dsub, a0, nl, off);
   if (storage == GFR_STORE_F16) {
     GFR_ZS(GFR_STORE_F16)
   } else if (storage == GFR_STORE_BF16) {
@@ -375,12 +381,31 @@ hipError_t zsum_launch(const DevGeom& G, const float* R, unsigned rbytes, const 
   return hipGetLastError();
 }
 
+// Deterministic sum of the first n(subset) partials; with dsub the count
+// follows the device-side subset index.
+__global__ void k_llh_sum(const double* __restrict__ part, int per_angle, int nang, int subset,
+                          int nsub, const int* __restrict__ dsub, double* __restrict__ out) {
+  if (dsub) subset = *dsub;
+  const int n = per_angle * ((nang - subset + nsub - 1) / nsub);
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += part[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
 }  // namespace
 
 hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r_cap_floats,
-                          const FwdOut& out, int subset, int nsub, const Tune& tune,
-                          hipStream_t st) {
-  const int nav = nav_of(G, subset, nsub);
+                          const FwdOut& out, int subset, int nsub, const int* dsub,
+                          const Tune& tune, hipStream_t st) {
+  // with a device-side subset index the launch covers the largest subset
+  const int nav = dsub ? nav_of(G, 0, nsub) : nav_of(G, subset, nsub);
   const int nl = G.l1 - G.l0 + 1;
   const size_t per = rotate_floats_per_angle(G);
   int chunk = (int)(r_cap_floats / per);
@@ -395,18 +420,26 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
     const int na = min(chunk, nav - a0);
     const unsigned rbytes = (unsigned)((size_t)na * per * sizeof(float));
     const unsigned grot = nrb * ((nl + 3) / 4) * G.nzs * na;
-    hipLaunchKernelGGL(k_rotate, dim3(grot), dim3(kWave, 4), 0, st, G, img, R, subset, nsub, a0,
-                       nl);
+    hipLaunchKernelGGL(k_rotate, dim3(grot), dim3(kWave, 4), 0, st, G, img, R, subset, nsub, dsub,
+                       a0, nl);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     switch (tune.zsum_sg) {
-      case 4: e = zsum_launch<4>(G, R, rbytes, o, ratio, out.storage, subset, nsub, a0, na, nl, st); break;
-      case 16: e = zsum_launch<16>(G, R, rbytes, o, ratio, out.storage, subset, nsub, a0, na, nl, st); break;
-      default: e = zsum_launch<8>(G, R, rbytes, o, ratio, out.storage, subset, nsub, a0, na, nl, st); break;
+      case 4: e = zsum_launch<4>(G, R, rbytes, o, ratio, out.storage, subset, nsub, dsub, a0, na, nl, st); break;
+      case 16: e = zsum_launch<16>(G, R, rbytes, o, ratio, out.storage, subset, nsub, dsub, a0, na, nl, st); break;
+      default: e = zsum_launch<8>(G, R, rbytes, o, ratio, out.storage, subset, nsub, dsub, a0, na, nl, st); break;
     }
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+hipError_t launch_llh_sum(const DevGeom& G, const double* part, int subset, int nsub,
+                          const int* dsub, int sg, double* out, hipStream_t st) {
+  const int per_angle = ((G.nrad + 127) / 128) * ((G.nsinogs + 4 * sg - 1) / (4 * sg));
+  hipLaunchKernelGGL(k_llh_sum, dim3(1), dim3(256), 0, st, part, per_angle, G.nang, subset, nsub,
+                     dsub, out);
+  return hipGetLastError();
 }
 
 size_t packed_bytes(const DevGeom& G, int subset, int nsub) {
@@ -422,11 +455,11 @@ hipError_t launch_pack(const DevGeom& G, const float* corr, const uint8_t* mask,
 }
 
 hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* numer, float* denom,
-                              int subset, int nsub, const Tune& tune, hipStream_t st) {
-  const int nav = nav_of(G, subset, nsub);
-  const size_t bytes = packed_bytes(G, subset, nsub);
+                              int subset, int nsub, const int* dsub, const Tune& tune,
+                              hipStream_t st) {
+  const size_t bytes = packed_bytes(G, dsub ? 0 : subset, nsub);
   if (bytes >= 0xFFFFFFF0u) return hipErrorInvalidValue;
-#define GFR_BK(ZB)                                                                                hipLaunchKernelGGL(k_back<ZB>, dim3((G.res + 15) / 16, (G.res + 15) / 16, (G.nzs + ZB - 1) / ZB),                      dim3(256), 0, st, G, corr2, (unsigned)bytes, numer, denom, subset, nsub, nav)
+#define GFR_BK(ZB)                                                                                hipLaunchKernelGGL(k_back<ZB>, dim3((G.res + 15) / 16, (G.res + 15) / 16, (G.nzs + ZB - 1) / ZB),                      dim3(256), 0, st, G, corr2, (unsigned)bytes, numer, denom, subset, nsub, dsub)
   switch (tune.back_zb) {
     case 4: GFR_BK(4); break;
     case 16: GFR_BK(16); break;
@@ -434,6 +467,70 @@ hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* nume
   }
 #undef GFR_BK
   return hipGetLastError();
+}
+
+int llh_capacity(const GeomHost& h, int nsub) {
+  const int nav = (h.g.nang + nsub - 1) / nsub;
+  return ((h.g.nrad + 127) / 128) * ((h.info.nsinogs + 15) / 16) * nav;  // zsum_sg = 4
+}
+
+hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float* img, float* R,
+                       size_t r_floats, const float2* corr2, float* num, float* den,
+                       hipStream_t st, Tune* out) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return hipErrorUnknown;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return hipErrorUnknown;
+  }
+  hipError_t e = hipSuccess;
+  // best of two timed runs after one warm-up
+  auto time_it = [&](auto&& launch) -> float {
+    float best = 1e30f;
+    for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+      e = hipEventRecord(e0, st);
+      if (e == hipSuccess) e = launch();
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && rep > 0)
+        best = ms < best ? ms : best;
+    }
+    return best;
+  };
+  Tune t;
+  const int cand[3] = {4, 8, 16};
+  float bz = 1e30f, bb = 1e30f;
+  for (int c : cand) {
+    Tune tt = t;
+    tt.zsum_sg = c;
+    const float ms = time_it([&] {
+      return launch_forward(G, img, R, r_floats, fo, 0, nsub, nullptr, tt, st);
+    });
+    if (ms < bz) {
+      bz = ms;
+      t.zsum_sg = c;
+    }
+  }
+  for (int c : cand) {
+    Tune tt = t;
+    tt.back_zb = c;
+    const float ms = time_it([&] {
+      return launch_back_packed(G, corr2, num, den, 0, nsub, nullptr, tt, st);
+    });
+    if (ms < bb) {
+      bb = ms;
+      t.back_zb = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return e;
+  t.tuned = true;
+  t.ms_zsum = bz;
+  t.ms_back = bb;
+  *out = t;
+  return hipSuccess;
 }
 
 }  // namespace gfr

@@ -56,11 +56,6 @@ namespace {
 
 size_t store_bytes(int storage) { return storage == GFR_STORE_F32 ? 4 : 2; }
 
-int llh_blocks(const GeomHost& h, int nsub) {
-  const int nav = (h.g.nang + nsub - 1) / nsub;
-  return ((h.g.nrad + 63) / 64) * ((h.info.nsinogs + 3) / 4) * nav;
-}
-
 struct Plan {
   size_t img, sino_f32, sino_store, mask, llh, corr2, rot;
   int nimg, nsino_f32, nsino_store;
@@ -78,7 +73,7 @@ Plan plan_of(const GeomHost& h, const gfr_recon_params* p, bool bgD, bool bgT) {
   pl.sino_f32 = sizeof(float) * (size_t)h.info.ndata;
   pl.sino_store = store_bytes(p->storage) * (size_t)h.info.ndata;
   pl.mask = (size_t)h.info.ndata;
-  pl.llh = sizeof(double) * (size_t)llh_blocks(h, p->nsub) + 64;
+  pl.llh = sizeof(double) * (size_t)llh_capacity(h, p->nsub) + 64;
   const size_t nav = (size_t)(h.g.nang + p->nsub - 1) / p->nsub;
   pl.corr2 = sizeof(float2) * nav * ((size_t)h.info.nsinogs + 2 * (size_t)h.g.nseg) * h.g.nrad;
   pl.rot = sizeof(float) * (size_t)(h.info.lambda1 - h.info.lambda0 + 1) * h.g.nzs * h.g.nrad *
@@ -131,9 +126,9 @@ struct PhaseMark {
   ~PhaseMark() { end(); }
 };
 
-// Times each tile shape of the two projectors on this session's data (subset
-// 0, doubles) and keeps the fastest; the result is shared with later sessions
-// of the same geometry and subset count. Any shape gives identical results.
+// Tile shapes for this session's geometry and subset count: measured on its
+// own buffers the first time (then shared with later sessions and batches of
+// the same geometry and subset count), defaults when GFR_AUTOTUNE=0.
 hipError_t autotune(gfr_recon* r) {
   GeomDev* gd = r->gd;
   const int S = r->p.nsub;
@@ -145,19 +140,11 @@ hipError_t autotune(gfr_recon* r) {
       return hipSuccess;
     }
   }
-  Tune t;
   const char* env = getenv("GFR_AUTOTUNE");
   if (env && env[0] == '0') {
-    r->tune = t;
+    r->tune = Tune();
     return hipSuccess;
   }
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess) return hipErrorUnknown;
-  if (hipEventCreate(&e1) != hipSuccess) {
-    (void)hipEventDestroy(e0);
-    return hipErrorUnknown;
-  }
-  const DevGeom& G = gd->dev;
   FwdOut fo;
   fo.corr2 = r->corr2;
   fo.mask = r->mask;
@@ -166,46 +153,10 @@ hipError_t autotune(gfr_recon* r) {
   fo.storage = r->p.storage;
   fo.yscale = (float)r->g.res;
   fo.llh_part = r->llh_part;
-  hipError_t e = hipSuccess;
-  auto time_it = [&](auto&& launch) -> float {
-    float best = 1e30f;
-    for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
-      e = hipEventRecord(e0, r->st);
-      if (e == hipSuccess) e = launch();
-      if (e == hipSuccess) e = hipEventRecord(e1, r->st);
-      if (e == hipSuccess) e = hipEventSynchronize(e1);
-      float ms = 0.f;
-      if (e == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && rep > 0)
-        best = ms < best ? ms : best;
-    }
-    return best;
-  };
-  const int cand[3] = {4, 8, 16};
-  float bz = 1e30f, bb = 1e30f;
-  for (int c : cand) {
-    Tune tt = t;
-    tt.zsum_sg = c;
-    const float ms = time_it([&] { return launch_forward(G, r->xb, r->R, r->r_floats, fo, 0, S, tt, r->st); });
-    if (ms < bz) {
-      bz = ms;
-      t.zsum_sg = c;
-    }
-  }
-  for (int c : cand) {
-    Tune tt = t;
-    tt.back_zb = c;
-    const float ms = time_it([&] { return launch_back_packed(G, r->corr2, r->num, r->den, 0, S, tt, r->st); });
-    if (ms < bb) {
-      bb = ms;
-      t.back_zb = c;
-    }
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  Tune t;
+  hipError_t e = tune_shapes(gd->dev, S, fo, r->xb, r->R, r->r_floats, r->corr2, r->num, r->den,
+                             r->st, &t);
   if (e != hipSuccess) return e;
-  t.tuned = true;
-  t.ms_zsum = bz;
-  t.ms_back = bb;
   r->tune = t;
   std::lock_guard<std::mutex> lk(r->ctx->mu);
   gd->tune[S] = t;
@@ -303,7 +254,7 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   e = launch_fill_cylinder(G, 1.f, r->xb, st);
   FwdOut fo;
   fo.proj = r->pb;
-  if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fo, 0, 1, r->tune, st);
+  if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fo, 0, 1, nullptr, r->tune, st);
   if (e == hipSuccess) e = hipMemcpyAsync(stage, norm, pl.sino_f32, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = launch_calib_mask(G, r->pb, stage, r->mask, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -335,7 +286,7 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
     if (e == hipSuccess) e = launch_blur_xy(G, t9, r->xT, r->xb, st);
     FwdOut fc;
     fc.proj = r->ptc;
-    if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fc, 0, 1, r->tune, st);
+    if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fc, 0, 1, nullptr, r->tune, st);
     if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create vlor cache"));
   }
   e = hipStreamSynchronize(st);
@@ -372,7 +323,7 @@ int gfr_recon_step(gfr_recon* r) {
       PhaseMark pm(r, GFR_PHASE_FORWARD);
       FwdOut fc;
       fc.proj = r->ptc;
-      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fc, s, S, r->tune, st));
+      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fc, s, S, nullptr, r->tune, st));
     }
     PhaseMark pm(r, GFR_PHASE_VLOR);
     int rc = gfr_vlor_assign(ctx, r->vl, r->ptc, r->dv, s, S, st);
@@ -393,16 +344,15 @@ int gfr_recon_step(gfr_recon* r) {
       fo.storage = p.storage;
       fo.yscale = (float)r->g.res;
       fo.llh_part = r->llh_part;
-      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fo, s, S, r->tune, st));
+      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fo, s, S, nullptr, r->tune, st));
     }
-    GFR_HIP(ctx, launch_llh_final(r->llh_part, forward_llh_blocks(G, s, S, r->tune.zsum_sg),
-                                  r->d_llh, st));
+    GFR_HIP(ctx, launch_llh_sum(G, r->llh_part, s, S, nullptr, r->tune.zsum_sg, r->d_llh, st));
     r->llh_valid = 1;
   }
   // Backprojection of the subset (GFIRST.cu:436-469) and update (:482-502).
   {
     PhaseMark pm(r, GFR_PHASE_BACK);
-    GFR_HIP(ctx, launch_back_packed(G, r->corr2, r->num, r->den, s, S, r->tune, st));
+    GFR_HIP(ctx, launch_back_packed(G, r->corr2, r->num, r->den, s, S, nullptr, r->tune, st));
   }
   {
     PhaseMark pm(r, GFR_PHASE_UPDATE);

@@ -279,6 +279,64 @@ int gfr_recon_profile(gfr_recon* r, int enable);
  * and pass counts per phase since the last call (arrays of GFR_NPHASES). */
 int gfr_recon_phase_times(gfr_recon* r, double* ms, long long* counts);
 
+/* ---------------- batched frames (one graph launch per subset step) ---------------- */
+
+/* Many doubles reconstructions (mode 1 of DRV:44-575) of one geometry, e.g.
+ * the frames of a dynamic study, each with its own data, subset count and
+ * iteration count. A step advances every active frame by one subset pass;
+ * the whole step is one captured hipGraph (rebuilt only when frames join or
+ * finish), with frames spread over `lanes` independent chains of scratch so
+ * their kernels overlap on the device. Each frame's result is bit-identical
+ * to a gfr_recon session with the same data and parameters. The reference
+ * runs one study per process. */
+typedef struct gfr_batch gfr_batch;
+typedef struct gfr_batch_params {
+  int storage;     /* GFR_STORE_* of every frame's sinograms */
+  int max_frames;  /* frame slots */
+  int min_nsub;    /* smallest subset count a frame may use (sizes the scratch) */
+  int lanes;       /* independent scratch chains, >= 1 */
+  float psf_fw, psf_bw;
+} gfr_batch_params;
+/* Device bytes of the batch without frames, and of one frame. */
+long long gfr_batch_footprint(const gfr_geometry* g, const gfr_batch_params* p);
+long long gfr_batch_frame_footprint(const gfr_geometry* g, const gfr_batch_params* p, int has_bg);
+/* norm: ndata floats (host); the calibration mask is shared by all frames. */
+int gfr_batch_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_batch_params* p,
+                     const float* norm, gfr_batch** out);
+/* Adds a frame: prompts (and background, or NULL) are ndata floats, unscaled,
+ * in host or device memory. Admission checks the context's HBM budget
+ * (GFR_ENOMEM when it does not fit). Returns the frame id in *frame. */
+int gfr_batch_add(gfr_batch* b, int niter, int nsub, const float* prompts, const float* bg,
+                  int* frame);
+/* One subset pass of every active frame; returns the number of frames still
+ * active afterwards (>= 0) or an error code. Enqueues only. */
+int gfr_batch_step(gfr_batch* b);
+/* Steps until every frame is complete, then waits. */
+int gfr_batch_run(gfr_batch* b);
+/* Frame position (as gfr_recon_cursor); returns 1 when the frame is complete. */
+int gfr_batch_cursor(gfr_batch* b, int frame, int* iter, int* subset, long long* passes);
+/* Copies a frame's image (nvox floats) to the host. */
+int gfr_batch_get(gfr_batch* b, int frame, float* host_x);
+/* FP64 log-likelihood of the frame's last pass. */
+int gfr_batch_loglik(gfr_batch* b, int frame, double* out);
+/* Frees a frame's slot and memory (waits for the batch's work first). */
+int gfr_batch_remove(gfr_batch* b, int frame);
+/* 1: steps replay a captured graph (default); 0: the same kernels are
+ * launched one by one. */
+int gfr_batch_use_graph(gfr_batch* b, int enable);
+/* Graph launches and graph (re)builds so far. */
+int gfr_batch_stats(gfr_batch* b, long long* launches, long long* builds);
+void* gfr_batch_stream(gfr_batch* b);
+int gfr_batch_destroy(gfr_batch* b);
+
+/* ---------------- synthetic measurements ---------------- */
+
+/* out[i] = Poisson(scale * mean[i]) as floats, seeded counter-based RNG
+ * (Philox4x32-10): the draw of bin i depends only on (seed, i). Device
+ * pointers; the reference reads measured files only. */
+int gfr_poisson(gfr_ctx* ctx, const float* d_mean, double scale, unsigned long long seed,
+                float* d_out, long long n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,23 @@ def test_footprint_scales_with_storage():
     assert f32 - f16 == 2 * 2 * C1.ndata  # prompts + background halve
 
 
+def test_batch_footprint():
+    bp = ga.batch_params(storage=ga.STORE_F16, max_frames=64, min_nsub=8, lanes=2)
+    base = ga.batch_footprint(C1, bp)
+    per_bg = ga.batch_footprint(C1, bp, 1, has_bg=True) - base
+    per = ga.batch_footprint(C1, bp, 1) - base
+    assert per == 4 * C1.nvox + 2 * C1.ndata and per_bg == per + 2 * C1.ndata
+    # 64 FP16 frames of the Inveon-size study fit comfortably in one GPU
+    assert ga.batch_footprint(C1, bp, 64, has_bg=True) < 32 << 30
+    # one more lane adds one scratch set; a larger min_nsub shrinks it
+    bp3 = ga.batch_params(storage=ga.STORE_F16, max_frames=64, min_nsub=8, lanes=3)
+    bp16 = ga.batch_params(storage=ga.STORE_F16, max_frames=64, min_nsub=16, lanes=2)
+    assert ga.batch_footprint(C1, bp3) > base > ga.batch_footprint(C1, bp16)
+    for bad in (ga.batch_params(lanes=0), ga.batch_params(min_nsub=0),
+                ga.batch_params(max_frames=0)):
+        assert ga.batch_footprint(C1, bad) == ga.GFR_EINVAL
+
+
 def test_read_sinogram_file_semantics(tmp_path, orc):
     nd = 12
     p = tmp_path / "s.raw"
