@@ -7,6 +7,21 @@ U[0.8, 1.2]. A step is one subset-iteration: PSF blur, forward projection +
 ratio (+ FP64 log-likelihood), backprojection and image update of one
 subset, all on the device with the data resident in HBM.
 
+--workload c3: the batched dynamic study. 64 frames of the same geometry,
+counts log-uniform in [1e5, 1e7], subset counts drawn from {8, 16, 24},
+activity = phantom channels weighted by a seeded two-compartment time curve,
+FP16 storage by default; every frame advances one subset per launch of one
+captured hipGraph until all are complete. Compared with the same frames run
+one study at a time as single sessions.
+
+--workload c4: the served stream. Requests arrive as a Poisson process (4/s
+for 300 s), 60% c0-size, 30% c1-size, 10% joint doubles+triples (VLOR) at
+c1 size, counts log-uniform in [1e5, 5e7], 2-4 iterations, FP16 storage,
+200 GB HBM budget; p50/p99 latency from submission to the image on the host.
+
+--copy-probe: a plain 1 GiB device copy (torch) repeated, for scaling the
+profiler's FETCH_SIZE / WRITE_SIZE counters to bytes.
+
 Prints one JSON line (stdout). Progress goes to stderr.
 """
 import argparse
@@ -22,6 +37,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2}  # GFR_STORE_*
 
 
 def log(msg):
@@ -92,25 +108,11 @@ def cpu_baseline(g, d2, norm, pb, nsub, nsteps):
                 seconds=dt)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=48)
-    ap.add_argument("--warmup", type=int, default=8)
-    ap.add_argument("--nsub", type=int, default=16)
-    ap.add_argument("--counts", type=float, default=5e7)
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
-    ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
+def run_c1(args):
     import gfirst_amd as ga
     from gfirst_amd.geometry import C1
     g = C1
-    storage = {"f32": ga.STORE_F32, "f16": ga.STORE_F16, "bf16": ga.STORE_BF16}[args.storage]
+    storage = STORAGE[args.storage or "f32"]
     log("geometry c1: nsinogs=%d ndata=%d nvox=%d" % (g.nsinogs, g.ndata, g.nvox))
     ctx = ga.Context(0)
     t = time.perf_counter()
@@ -167,7 +169,7 @@ def main():
         "data": "synthetic (seeded mouse phantom, Poisson 5e7 counts, norm U[0.8,1.2])",
         "config": {"workload": "c1 Inveon-scale doubles OSEM: image 128x128x159, sinogram "
                                "128x160x4319, span 3, MRD 79, %d subsets" % args.nsub,
-                   "subsets": args.nsub, "storage": args.storage},
+                   "subsets": args.nsub, "storage": args.storage or "f32"},
         "hbm_gbps_alg": round(ab["total"] / (ms_step * 1e-3) / 1e9, 2),
         "phase_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in ph.items() if v[1]},
         "roofline": {"kernel": "k_forward" if dom == "forward" else "k_back", "bound": "hbm",
@@ -187,6 +189,425 @@ def main():
         res["cpu_baseline"] = cpu_baseline(g, d2, norm, pb, args.nsub, args.cpu_steps)
         log("cpu: %.3f subset-iters/s on %d threads" % (res["cpu_baseline"]["value"],
                                                        res["cpu_baseline"]["cores"]))
+    return res
+
+
+
+def c3_inputs(ctx, g, seed):
+    """Expected sinograms of the two phantom channels (PSF-blurred, projected
+    on the device) and their totals."""
+    import gfirst_amd as ga  # noqa: F401
+    from gfirst_amd import synth
+    P, tot = [], []
+    for ch in (0, 1):
+        img = ctx.to_device(synth.phantom_mouse(g, seed=seed, channel=ch))
+        blur = ctx.array(g.nvox)
+        ctx.blur_xy(g, img, 1.5, out=blur)
+        p = ctx.array(g.ndata)
+        ctx.forward_project(g, blur, out=p)
+        ctx.sync()
+        tot.append(float(p.download().sum(dtype=np.float64)))
+        P.append(p)
+    return P, tot
+
+
+def frame_scales(f, tot):
+    """Scales of the two channel projections giving the frame's activity mix
+    and total expected counts."""
+    mix = f["cp"] * tot[0] + f["ct"] * tot[1]
+    return f["counts"] * f["cp"] / mix, f["counts"] * f["ct"] / mix
+
+
+def c3_fill(ctx, bt, frames, P, tot, niter, ybuf, seed):
+    for k, f in enumerate(frames):
+        a, b = frame_scales(f, tot)
+        ctx.poisson(P[0], a, seed * 1000003 + k, out=ybuf, mean2=P[1], scale2=b)
+        ctx.sync()  # the batch copies on its own stream
+        bt.add(niter, f["nsub"], ybuf)
+
+
+def equiv_passes(g, nsub, passes):
+    """Subset passes expressed in passes of a 16-subset split (work scales
+    with the angles per subset)."""
+    return passes * ((g.nang + nsub - 1) // nsub) / ((g.nang + 15) // 16)
+
+
+def run_c3(args):
+    import gfirst_amd as ga
+    from gfirst_amd import synth
+    from gfirst_amd.geometry import C1
+    g = C1
+    sname = args.storage or "f16"
+    storage = STORAGE[sname]
+    frames = synth.dynamic_study(args.frames, seed=args.seed)
+    niter = args.c3_niter
+    ctx = ga.Context(0)
+    t = time.perf_counter()
+    P, tot = c3_inputs(ctx, g, args.seed)
+    norm = synth.norm_uniform(g.ndata, seed=args.seed + 2)
+    ybuf = ctx.array(g.ndata)
+    log("c3: %d frames, subsets %s, counts %.3g..%.3g, channel projections in %.1fs" % (
+        len(frames), sorted(set(f["nsub"] for f in frames)), min(f["counts"] for f in frames),
+        max(f["counts"] for f in frames), time.perf_counter() - t))
+
+    def make_batch(lanes, sub):
+        bp = ga.batch_params(storage=storage, max_frames=len(sub), min_nsub=min(f["nsub"] for f in sub),
+                             lanes=lanes)
+        bt = ctx.batch(g, bp, norm)
+        c3_fill(ctx, bt, sub, P, tot, niter, ybuf, args.seed)
+        return bt
+
+    # lanes: measured on a short run of the first frames unless given
+    lanes = args.lanes
+    probe = {}
+    if lanes <= 0:
+        sub = frames[:min(16, len(frames))]
+        for L in (1, 2, 3, 4):
+            bt = make_batch(L, sub)
+            bt.step()
+            bt.sync()
+            t0 = time.perf_counter()
+            n = 0
+            for _ in range(6):
+                n += sum(1 for k in range(len(sub)) if not bt.cursor(k)["done"])
+                bt.step()
+            bt.sync()
+            probe[L] = n / (time.perf_counter() - t0)
+            bt.close()
+        lanes = max(probe, key=probe.get)
+        log("lanes probe (passes/s): %s -> %d" % ({k: round(v, 1) for k, v in probe.items()}, lanes))
+    bt = make_batch(lanes, frames)
+    log("batch resident: %d frames, %.2f GB in use" % (len(frames), ctx.bytes_in_use() / 1e9))
+    ctx.sync()
+    t0 = time.perf_counter()
+    nsteps = 0
+    while bt.step() > 0:
+        nsteps += 1
+    nsteps += 1
+    bt.sync()
+    dt = time.perf_counter() - t0
+    passes = sum(bt.cursor(k)["passes"] for k in range(len(frames)))
+    eq = sum(equiv_passes(g, f["nsub"], niter * f["nsub"]) for f in frames)
+    st = bt.stats()
+    lls = [bt.loglik(k) for k in range(len(frames))]
+    bt.close()
+    log("batched: %d frames, %d passes in %.3fs over %d graph launches (%d builds): "
+        "%.1f passes/s, %.1f S=16-equivalent subset-iters/s" % (
+            len(frames), passes, dt, st["launches"], st["builds"], passes / dt, eq / dt))
+    # plain single-study path: the first frames as separate sessions, one at a time
+    pl_pass = pl_eq = pl_t = 0.0
+    nplain = min(args.plain_frames, len(frames))
+    for k in range(nplain):
+        f = frames[k]
+        a, b = frame_scales(f, tot)
+        ctx.poisson(P[0], a, args.seed * 1000003 + k, out=ybuf, mean2=P[1], scale2=b)
+        ctx.sync()
+        d2 = synth.as_file_pair(ybuf.download())
+        r = ctx.recon(g, ga.recon_params(mode=1, niter=niter, nsub=f["nsub"], storage=storage),
+                      d2, None, norm)
+        r.sync()
+        t1 = time.perf_counter()
+        r.run()
+        pl_t += time.perf_counter() - t1
+        if r.loglik() != lls[k]:
+            raise RuntimeError("frame %d: batched and single-session results differ" % k)
+        r.close()
+        pl_pass += niter * f["nsub"]
+        pl_eq += equiv_passes(g, f["nsub"], niter * f["nsub"])
+    res = {
+        "metric": "Inveon-size OSEM subset-iters/s (batched dynamic study)",
+        "value": round(eq / dt, 3),
+        "unit": "subset-iters/s (S=16-equivalent, all frames)",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "warmup": 0,
+        "ms_per_step": round(1e3 * dt / nsteps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "%s-storage/f32" % sname if sname != "f32" else "f32",
+        "data": "synthetic (two seeded mouse-phantom channels x two-compartment curves, "
+                "device Poisson, norm U[0.8,1.2])",
+        "config": {"workload": "c3 batched dynamic study: %d frames of the Inveon-scale "
+                               "geometry, subsets {8,16,24}, %d iterations, counts "
+                               "log-U[1e5,1e7]" % (len(frames), niter),
+                   "frames": len(frames), "lanes": lanes, "storage": sname},
+        "passes": passes,
+        "passes_per_s": round(passes / dt, 2),
+        "seconds": round(dt, 3),
+        "graph": st,
+        "lanes_probe": {str(k): round(v, 2) for k, v in probe.items()},
+        "plain_sessions": {"frames": nplain, "seconds": round(pl_t, 3),
+                           "subset_iters_per_s_equiv": round(pl_eq / pl_t, 3) if pl_t else None,
+                           "same_loglik_as_batched": True},
+        "batched_over_plain": round((eq / dt) / (pl_eq / pl_t), 3) if pl_t else None,
+    }
+    if args.error_report:
+        res["error_vs_fp64"] = c3_error_report(ctx, g, frames[0], P, tot, norm, ybuf, args.seed)
+    ctx.close()
+    return res
+
+
+def c3_error_report(ctx, g, f, P, tot, norm, ybuf, seed):
+    """Image error of one frame (one iteration) in each storage format against
+    the FP64 oracle on the same data."""
+    import gfirst_amd as ga
+    from gfirst_amd import synth
+    from oracle import oracle as orc
+    a, b = frame_scales(f, tot)
+    ctx.poisson(P[0], a, seed * 1000003, out=ybuf, mean2=P[1], scale2=b)
+    ctx.sync()
+    d2 = synth.as_file_pair(ybuf.download())
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    orc.set_threads(threads)
+    t0 = time.perf_counter()
+    x64 = oracle_fp64_osem1(orc, g, d2, norm, f["nsub"])
+    log("FP64 oracle: 1 iteration S=%d in %.1fs" % (f["nsub"], time.perf_counter() - t0))
+    x64 = np.asarray(x64, np.float64)
+    out = {"frame": 0, "nsub": f["nsub"], "iterations": 1, "counts": round(f["counts"]),
+           "max_count": float(d2[:g.ndata].max())}
+    for name, st in STORAGE.items():
+        r = ctx.recon(g, ga.recon_params(mode=1, niter=1, nsub=f["nsub"], storage=st), d2, None,
+                      norm)
+        r.run()
+        x = r.get(ga.OUT_XD).astype(np.float64)
+        r.close()
+        out[name] = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
+        log("rel L2 vs FP64 oracle, %s storage: %.3g" % (name, out[name]))
+    return out
+
+
+def oracle_fp64_osem1(orc, g, d2, norm, S, psf=1.5):
+    """One doubles iteration (S subsets) of the oracle in FP64, pass by pass
+    with progress lines: mask from the projected cylinder and the norm
+    (GFIRST.cu:155-181, 453-458), then blur, forward, ratio, back, update per
+    subset (GFIRST.cu:324-502)."""
+    nd = g.ndata
+    cyl = orc.cylinder(g, 1.0, fp64=True)
+    pb = np.zeros(nd, np.float64)
+    for s in range(16):
+        orc.forward(g, cyl, s, 16, out=pb, fp64=True)
+    log("FP64 oracle: PROJ_BLANK done")
+    cal = orc.calib(g, pb, norm, fp64=True)
+    del pb
+    y = float(g.res) * d2[:nd].astype(np.float64)
+    x = cyl
+    for s in range(S):
+        xb = orc.blur_xy(g, x, psf, fp64=True)
+        proj = orc.forward(g, xb, s, S, fp64=True)
+        corr = orc.ratio(g, proj, y, None, s, S, fp64=True)
+        del proj
+        num, den = orc.back(g, corr, cal, s, S, fp64=True)
+        del corr
+        x = orc.update(g, x, num, den, psf, fp64=True)
+        log("FP64 oracle: subset %d/%d" % (s + 1, S))
+    return x
+
+
+C4_CLASSES = (("c0", 0.6), ("c1", 0.3), ("c2", 0.1))
+
+
+def c4_setup(ctx, seed):
+    """Per request class: geometry, expected sinograms on the device, the
+    normalization (device) and, for the joint class, VLOR records."""
+    import gfirst_amd as ga  # noqa: F401
+    from gfirst_amd import listmode, synth
+    from gfirst_amd.geometry import C0, C1
+
+    def project(g, img):
+        d = ctx.to_device(img)
+        b = ctx.array(g.nvox)
+        ctx.blur_xy(g, d, 1.5, out=b)
+        p = ctx.array(g.ndata)
+        ctx.forward_project(g, b, out=p)
+        ctx.sync()
+        return p, float(p.download().sum(dtype=np.float64))
+
+    cls = {}
+    y0, s0 = project(C0, synth.phantom_c0(C0, seed=seed))
+    cls["c0"] = dict(g=C0, nsub=4, mode=1, y=y0, ysum=s0,
+                     norm=ctx.to_device(np.ones(C0.ndata, np.float32)))
+    n1 = ctx.to_device(synth.norm_uniform(C1.ndata, seed=seed + 2))
+    y1, s1 = project(C1, synth.phantom_mouse(C1, seed=seed, channel=0))
+    cls["c1"] = dict(g=C1, nsub=16, mode=1, y=y1, ysum=s1, norm=n1)
+    chB = synth.phantom_mouse(C1, seed=seed, channel=1)
+    yT, sT = project(C1, chB)
+    t0 = time.perf_counter()
+    hits = listmode.triple_events(C1, chB, 2000000, seed=seed + 5)
+    rec = listmode.bin_hits(C1, hits, 80.5)
+    rec = rec[(rec[:, 0] > 0) & (rec[:, 1] > 0)]
+    log("c4: %d VLOR records binned in %.1fs" % (len(rec), time.perf_counter() - t0))
+    cls["c2"] = dict(g=C1, nsub=16, mode=3, y=y1, ysum=s1, norm=n1, yT=yT, yTsum=sT, rec=rec)
+    return cls
+
+
+def c4_inputs(ctx, c, counts, seed):
+    g = c["g"]
+    d2 = ctx.array(2 * g.ndata).zero()
+    ctx.poisson(c["y"], counts / c["ysum"], seed, out=d2)
+    t2 = None
+    if c["mode"] >= 2:
+        t2 = ctx.array(2 * g.ndata).zero()
+        ctx.poisson(c["yT"], counts / 50.0 / c["yTsum"], seed + 1, out=t2)
+    ctx.sync()
+    return d2, t2
+
+
+def run_c4(args):
+    import gfirst_amd as ga
+    sname = args.storage or "f16"
+    storage = STORAGE[sname]
+    ctx = ga.Context(0, hbm_budget=int(args.budget_gb * 1e9))
+    t = time.perf_counter()
+    cls = c4_setup(ctx, args.seed)
+    log("c4: classes ready in %.1fs" % (time.perf_counter() - t))
+    rng = np.random.default_rng(args.seed + 11)
+    arrivals = []
+    ta = 0.0
+    while True:
+        ta += rng.exponential(1.0 / args.rate)
+        if ta > args.duration:
+            break
+        name = rng.choice([c for c, _ in C4_CLASSES], p=[w for _, w in C4_CLASSES])
+        counts = float(np.exp(rng.uniform(np.log(1e5), np.log(5e7))))
+        niter = int(rng.integers(2, 5))
+        arrivals.append((ta, str(name), counts, niter))
+    log("c4: %d arrivals over %.0fs (rate %.1f/s)" % (len(arrivals), args.duration, args.rate))
+    # untimed warm-up: one request of each class (tile shapes, code paths)
+    srv = ctx.server(max_active=args.max_active, inflight=2)
+    for k, name in enumerate(cls):
+        c = cls[name]
+        d2, t2 = c4_inputs(ctx, c, 1e6, 10 ** 6 + k)
+        p = ga.recon_params(mode=c["mode"], niter=1, nsub=c["nsub"], storage=storage)
+        rid = srv.submit(c["g"], p, d2, t2, c["norm"], c.get("rec"))
+        srv.wait(rid)
+        srv.release(rid)
+    ids = []
+    t0 = time.perf_counter()
+    for k, (ta, name, counts, niter) in enumerate(arrivals):
+        dt = t0 + ta - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        c = cls[name]
+        d2, t2 = c4_inputs(ctx, c, counts, args.seed * 7919 + k)
+        p = ga.recon_params(mode=c["mode"], niter=niter, nsub=c["nsub"], storage=storage)
+        ids.append((srv.submit(c["g"], p, d2, t2, c["norm"], c.get("rec")), name, niter, counts))
+        if k % 40 == 0:
+            st = srv.stats()
+            log("c4: t=%.1fs submitted %d, active %d, pending %d, done %d" % (
+                time.perf_counter() - t0, k + 1, st["active"], st["pending"], st["done"]))
+    for rid, _, _, _ in ids:
+        srv.wait(rid)
+    t_end = time.perf_counter()
+    infos = [(srv.info(rid), name, niter, counts) for rid, name, niter, counts in ids]
+    failed = [i for i, _, _, _ in infos if i["state"] != ga.REQ_DONE]
+    lat = {}
+    passes = eq = 0.0
+    for i, name, niter, counts in infos:
+        if i["state"] != ga.REQ_DONE:
+            continue
+        lat.setdefault(name, []).append(i["t_done"] - i["t_submit"])
+        c = cls[name]
+        npass = niter * c["nsub"] * (2 if c["mode"] >= 2 else 1)
+        passes += npass
+        eq += equiv_passes(c["g"], c["nsub"], npass) * (c["g"].ndata / cls["c1"]["g"].ndata)
+    allat = np.concatenate([np.asarray(v) for v in lat.values()])
+    span = max(i["t_done"] for i, _, _, _ in infos) - min(i["t_submit"] for i, _, _, _ in infos)
+    q = lambda a, x: round(float(np.percentile(a, x)) * 1e3, 2)  # noqa: E731
+    # bit-exact check of the first request of each class against a lone session
+    same = {}
+    for name in cls:
+        k = next((j for j, (_, nm, _, _) in enumerate(ids) if nm == name), None)
+        if k is None:
+            continue
+        rid, _, niter, counts = ids[k]
+        c = cls[name]
+        d2, t2 = c4_inputs(ctx, c, counts, args.seed * 7919 + k)
+        p = ga.recon_params(mode=c["mode"], niter=niter, nsub=c["nsub"], storage=storage)
+        r = ctx.recon(c["g"], p, d2, t2, c["norm"], c.get("rec"))
+        r.run()
+        xD, _ = srv.result(rid)
+        same[name] = bool(np.array_equal(r.get(ga.OUT_XD), xD))
+        r.close()
+    srv.close()
+    res = {
+        "metric": "p50/p99 recon latency under load",
+        "value": q(allat, 50),
+        "unit": "ms (p50 request latency)",
+        "n_gpus": 1,
+        "steps": len(ids),
+        "warmup": len(cls),
+        "ms_per_step": round(1e3 * span / max(len(ids), 1), 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "%s-storage/f32" % sname,
+        "data": "synthetic (seeded phantoms, device Poisson, Poisson arrivals)",
+        "config": {"workload": "c4 served stream: Poisson arrivals %.1f/s for %.0fs, mix 60%% "
+                               "c0 / 30%% c1 / 10%% c2 (mode 3, 2e6 VLOR events), counts "
+                               "log-U[1e5,5e7], 2-4 iterations, %.0f GB budget" % (
+                                   args.rate, args.duration, args.budget_gb),
+                   "max_active": args.max_active, "storage": sname},
+        "latency_ms": {"p50": q(allat, 50), "p99": q(allat, 99), "max": q(allat, 100),
+                       "per_class": {k: {"n": len(v), "p50": q(v, 50), "p99": q(v, 99)}
+                                     for k, v in lat.items()}},
+        "requests": len(ids), "failed": len(failed),
+        "subset_iters_per_s": round(passes / span, 2),
+        "c1_equiv_subset_iters_per_s": round(eq / span, 2),
+        "makespan_s": round(span, 2),
+        "drain_s": round(t_end - t0 - args.duration, 2),
+        "alone_equals_mix": same,
+    }
+    ctx.close()
+    return res
+
+
+def copy_probe(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"copy_bytes": nbytes, "reps": reps}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="c1", choices=["c1", "c3", "c4"])
+    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--nsub", type=int, default=16)
+    ap.add_argument("--counts", type=float, default=5e7)
+    ap.add_argument("--storage", default=None, choices=["f32", "f16", "bf16"])
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--frames", type=int, default=64)
+    ap.add_argument("--c3-niter", type=int, default=4)
+    ap.add_argument("--lanes", type=int, default=0, help="0: measure 1..4")
+    ap.add_argument("--plain-frames", type=int, default=6)
+    ap.add_argument("--error-report", action="store_true")
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--duration", type=float, default=300.0)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--max-active", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if args.copy_probe:
+        res = copy_probe()
+    elif args.workload == "c3":
+        res = run_c3(args)
+    elif args.workload == "c4":
+        res = run_c4(args)
+    else:
+        res = run_c1(args)
     print(json.dumps(res), flush=True)
 
 

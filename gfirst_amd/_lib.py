@@ -44,6 +44,21 @@ class GfrBatchParams(C.Structure):
                 ("lanes", C.c_int), ("psf_fw", C.c_float), ("psf_bw", C.c_float)]
 
 
+class GfrRequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int), ("error", C.c_int), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("passes", C.c_longlong),
+                ("bytes", C.c_longlong)]
+
+
+class GfrServerTotals(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("pending", C.c_longlong), ("active", C.c_longlong),
+                ("done", C.c_longlong), ("failed", C.c_longlong), ("passes", C.c_longlong),
+                ("now", C.c_double)]
+
+
+REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
 class GfrError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("gfr error %d: %s" % (code, msg))
@@ -124,7 +139,16 @@ SIGNATURES = {
     "gfr_batch_stats": (_I, [_V, C.POINTER(_LL), C.POINTER(_LL)]),
     "gfr_batch_stream": (_V, [_V]),
     "gfr_batch_destroy": (_I, [_V]),
-    "gfr_poisson": (_I, [_V, _V, C.c_double, C.c_ulonglong, _V, _LL, _V]),
+    "gfr_server_create": (_I, [_V, _I, _I, C.POINTER(_V)]),
+    "gfr_server_submit": (_I, [_V, _G, C.POINTER(GfrReconParams), _V, _V, _V, _V, _LL,
+                               C.POINTER(_LL)]),
+    "gfr_server_info": (_I, [_V, _LL, C.POINTER(GfrRequestInfo)]),
+    "gfr_server_wait": (_I, [_V, _LL, C.c_double]),
+    "gfr_server_result": (_I, [_V, _LL, _V, _V]),
+    "gfr_server_release": (_I, [_V, _LL]),
+    "gfr_server_stats": (_I, [_V, C.POINTER(GfrServerTotals)]),
+    "gfr_server_destroy": (_I, [_V]),
+    "gfr_poisson": (_I, [_V, _V, C.c_double, _V, C.c_double, C.c_ulonglong, _V, _LL, _V]),
 }
 
 PHASES = ("blur", "forward", "vlor", "back", "update", "jbf")

@@ -230,12 +230,20 @@ class Context:
     def batch(self, g, params, norm):
         return Batch(self, g, params, norm)
 
-    def poisson(self, mean, scale, seed, out=None):
-        """Seeded Poisson(scale * mean) draws on the device (float counts)."""
+    def server(self, max_active=32, inflight=2):
+        return Server(self, max_active, inflight)
+
+    def poisson(self, mean, scale, seed, out=None, mean2=None, scale2=0.0):
+        """Seeded Poisson(scale * mean + scale2 * mean2) draws on the device
+        (float counts)."""
         m, _ = self._dev(mean)
+        m2, _ = self._dev(mean2)
+        if m2 is not None and m2.n != m.n:
+            raise ValueError("mean2 must match mean")
         o = out if out is not None else self.array(m.n)
-        _check(self.h, lib().gfr_poisson(self.h, m.ptr, float(scale), int(seed), o.ptr, m.n,
-                                         None))
+        _check(self.h, lib().gfr_poisson(self.h, m.ptr, float(scale),
+                                         m2.ptr if m2 is not None else None, float(scale2),
+                                         int(seed), o.ptr, m.n, None))
         if out is not None:
             return o
         self.sync()
@@ -453,6 +461,104 @@ class Batch:
         if self.h:
             lib().gfr_batch_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop over this context (gfr_server_*): submit requests at any
+    time; each is admitted as a device-resident session when a slot and HBM
+    budget allow and advances pass by pass beside the others. Inputs may be
+    numpy arrays or DeviceArrays; they are kept referenced here until the
+    request has been admitted."""
+
+    def __init__(self, ctx, max_active=32, inflight=2):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(ctx.h, lib().gfr_server_create(ctx.h, max_active, inflight, C.byref(h)))
+        self.h = h.value
+        self._inputs = {}
+        self._meta = {}
+
+    def _ptr(self, a, n, keep, dtype=np.float32):
+        if a is None:
+            return None
+        if isinstance(a, DeviceArray):
+            if a.n < n:
+                raise ValueError("device input too small")
+            keep.append(a)
+            return a.ptr
+        a = np.ascontiguousarray(a, dtype).reshape(-1)
+        if a.size < n:
+            raise ValueError("input too small")
+        keep.append(a)
+        return a.ctypes.data
+
+    def submit(self, g, params, d2, t2, norm, records=None):
+        keep = []
+        nd = g.ndata
+        p_d2 = self._ptr(d2, 2 * nd, keep)
+        p_t2 = self._ptr(t2, 2 * nd, keep)
+        p_n = self._ptr(norm, nd, keep)
+        nrec = 0
+        p_r = None
+        if records is not None:
+            rec = np.ascontiguousarray(records, np.int32).reshape(-1)
+            keep.append(rec)
+            p_r = rec.ctypes.data
+            nrec = rec.size // 2
+        rid = C.c_longlong()
+        _check(self.ctx.h, lib().gfr_server_submit(self.h, C.byref(g.c()), C.byref(params), p_d2,
+                                                   p_t2, p_n, p_r, nrec, C.byref(rid)))
+        self._inputs[rid.value] = keep
+        self._meta[rid.value] = (g, params.mode)
+        self._drop_admitted()
+        return rid.value
+
+    def _drop_admitted(self):
+        for rid in list(self._inputs):
+            if self.info(rid)["state"] != _lib.REQ_PENDING:
+                del self._inputs[rid]
+
+    def info(self, rid):
+        i = _lib.GfrRequestInfo()
+        _check(self.ctx.h, lib().gfr_server_info(self.h, rid, C.byref(i)))
+        return dict(state=i.state, error=i.error, t_submit=i.t_submit, t_admit=i.t_admit,
+                    t_done=i.t_done, passes=i.passes, bytes=i.bytes)
+
+    def wait(self, rid, timeout=-1.0):
+        st = _check(self.ctx.h, lib().gfr_server_wait(self.h, rid, float(timeout)))
+        self._drop_admitted()
+        return st
+
+    def result(self, rid):
+        """(xD, xT or None) of a done request; raises GfrError for a failed one."""
+        g, mode = self._meta[rid]
+        xD = np.empty(g.nvox, np.float32)
+        xT = np.empty(g.nvox, np.float32) if mode >= 2 else None
+        _check(self.ctx.h, lib().gfr_server_result(self.h, rid, xD.ctypes.data,
+                                                   xT.ctypes.data if xT is not None else None))
+        return xD, xT
+
+    def release(self, rid):
+        _check(self.ctx.h, lib().gfr_server_release(self.h, rid))
+        self._meta.pop(rid, None)
+        self._inputs.pop(rid, None)
+
+    def stats(self):
+        t = _lib.GfrServerTotals()
+        _check(self.ctx.h, lib().gfr_server_stats(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def close(self):
+        if self.h:
+            lib().gfr_server_destroy(self.h)
+            self.h = None
+            self._inputs.clear()
 
     def __del__(self):
         try:

@@ -13,8 +13,19 @@
 namespace gfr {
 
 int set_err(gfr_ctx* ctx, int code, const std::string& msg) {
-  if (ctx) ctx->err = msg;
+  if (ctx) {
+    std::lock_guard<std::mutex> lk(ctx->err_mu);
+    ctx->err = msg;
+  }
   return code;
+}
+
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t at{};
+  const bool dev = hipPointerGetAttributes(&at, p) == hipSuccess && at.type == hipMemoryTypeDevice;
+  (void)hipGetLastError();  // a pageable host pointer reports an error: not sticky
+  return dev;
 }
 
 int hip_err(gfr_ctx* ctx, hipError_t e, const char* where) {
@@ -218,7 +229,14 @@ int gfr_ctx_destroy(gfr_ctx* ctx) {
   return GFR_OK;
 }
 
-const char* gfr_last_error(gfr_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+const char* gfr_last_error(gfr_ctx* ctx) {
+  if (!ctx) return "no context";
+  // copy per calling thread: valid until that thread's next call
+  static thread_local std::string copy;
+  std::lock_guard<std::mutex> lk(ctx->err_mu);
+  copy = ctx->err;
+  return copy.c_str();
+}
 
 void* gfr_ctx_stream(gfr_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
@@ -247,11 +265,13 @@ int gfr_alloc(gfr_ctx* ctx, size_t bytes, void** dptr) {
 int gfr_free(gfr_ctx* ctx, void* dptr) {
   GFR_CHECK_CTX(ctx);
   if (!dptr) return GFR_OK;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  auto it = ctx->allocs.find(dptr);
-  if (it == ctx->allocs.end()) return set_err(ctx, GFR_EINVAL, "gfr_free: unknown pointer");
-  ctx->in_use -= it->second;
-  ctx->allocs.erase(it);
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->allocs.find(dptr);
+    if (it == ctx->allocs.end()) return set_err(ctx, GFR_EINVAL, "gfr_free: unknown pointer");
+    ctx->in_use -= it->second;
+    ctx->allocs.erase(it);
+  }
   GFR_HIP(ctx, hipFree(dptr));
   return GFR_OK;
 }

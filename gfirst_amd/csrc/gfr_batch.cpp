@@ -329,10 +329,7 @@ int gfr_batch_add(gfr_batch* b, int niter, int nsub, const float* prompts, const
   const long long nd = h.info.ndata;
   // A background given in host memory that is all zero is dropped (as
   // gfr_recon_create does); device-memory backgrounds are kept as given.
-  hipPointerAttribute_t at{};
-  bool bg_dev = false;
-  if (bg && hipPointerGetAttributes(&at, bg) == hipSuccess) bg_dev = at.type == hipMemoryTypeDevice;
-  (void)hipGetLastError();
+  const bool bg_dev = is_device_ptr(bg);
   const bool has_bg = bg && (bg_dev || any_positive(bg, nd));
   const Sizes z = sizes_of(h, &b->p);
   const size_t need = has_bg ? z.frame_bg : z.frame_nobg;
@@ -541,12 +538,14 @@ int gfr_batch_destroy(gfr_batch* b) {
   return GFR_OK;
 }
 
-int gfr_poisson(gfr_ctx* ctx, const float* d_mean, double scale, unsigned long long seed,
-                float* d_out, long long n, void* stream) {
+int gfr_poisson(gfr_ctx* ctx, const float* d_mean, double scale, const float* d_mean2,
+                double scale2, unsigned long long seed, float* d_out, long long n, void* stream) {
   if (!ctx) return GFR_EINVAL;
   if ((!d_mean || !d_out) && n > 0) return set_err(ctx, GFR_EINVAL, "gfr_poisson: NULL pointer");
-  if (n < 0 || !(scale >= 0.0)) return set_err(ctx, GFR_EINVAL, "gfr_poisson: bad size or scale");
-  GFR_HIP(ctx, launch_poisson(d_mean, scale, (uint64_t)seed, d_out, n, stream_of(ctx, stream)));
+  if (n < 0 || !(scale >= 0.0) || !(scale2 >= 0.0))
+    return set_err(ctx, GFR_EINVAL, "gfr_poisson: bad size or scale");
+  GFR_HIP(ctx, launch_poisson(d_mean, scale, d_mean2, scale2, (uint64_t)seed, d_out, n,
+                              stream_of(ctx, stream)));
   return GFR_OK;
 }
 

@@ -95,6 +95,7 @@ struct gfr_ctx {
   size_t in_use = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  std::mutex err_mu;  // err is written by any thread using the context
   std::map<std::string, gfr::GeomDev*> geoms;
   std::map<void*, size_t> allocs;
   unsigned int* d_scratch_u32 = nullptr;  // 256 words for small reductions
@@ -117,6 +118,8 @@ namespace gfr {
 // Error helpers used by every entry point.
 int set_err(gfr_ctx* ctx, int code, const std::string& msg);
 int hip_err(gfr_ctx* ctx, hipError_t e, const char* where);
+// True for device memory (host and NULL pointers: false).
+bool is_device_ptr(const void* p);
 #define GFR_HIP(ctx, call)                                   \
   do {                                                       \
     hipError_t _e = (call);                                  \
@@ -184,11 +187,13 @@ hipError_t launch_scale_copy(const float* in, float s, float* out, long long n, 
 hipError_t launch_convert_store(const float* in, void* out, int storage, long long n,
                                 hipStream_t st);
 hipError_t launch_llh_final(const double* part, int n, double* out, hipStream_t st);
+// *flag = 1 if some v[i] is not <= 0 (NaN counts); *flag must start at 0.
+hipError_t launch_any_positive(const float* v, long long n, unsigned int* flag, hipStream_t st);
 // sub[slots[i]] = (sub[slots[i]] + 1) mod nsub[slots[i]] for i < n.
 hipError_t launch_advance(int* sub, const int* nsub, const int* slots, int n, hipStream_t st);
 // Synthetic measurements (gfr_synth.hip).
-hipError_t launch_poisson(const float* mean, double scale, uint64_t seed, float* out, long long n,
-                          hipStream_t st);
+hipError_t launch_poisson(const float* mean, double scale, const float* mean2, double scale2,
+                          uint64_t seed, float* out, long long n, hipStream_t st);
 
 // Largest number of FP64 partials a forward pass with `nsub` subsets writes
 // (any tile shape).

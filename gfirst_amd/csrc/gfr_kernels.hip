@@ -357,6 +357,25 @@ __global__ void k_advance(int* __restrict__ sub, const int* __restrict__ nsub,
 }
 }  // namespace
 
+namespace {
+__global__ void k_any_positive(const float* __restrict__ v, long long n,
+                               unsigned int* __restrict__ flag) {
+  bool any = false;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    any |= !(v[i] <= 0.f);
+  if (any) *flag = 1u;
+}
+}  // namespace
+
+hipError_t launch_any_positive(const float* v, long long n, unsigned int* flag, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_any_positive, dim3((unsigned)blocks), dim3(256), 0, st, v, n, flag);
+  return hipGetLastError();
+}
+
 hipError_t launch_advance(int* sub, const int* nsub, const int* slots, int n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_advance, dim3((n + 255) / 256), dim3(256), 0, st, sub, nsub, slots, n);

@@ -163,6 +163,25 @@ hipError_t autotune(gfr_recon* r) {
   return hipSuccess;
 }
 
+// Whether a background half holds any value that is not <= 0 (NaN counts),
+// for host or device memory.
+int bg_present(gfr_ctx* ctx, const float* v, long long n, bool* out) {
+  if (!is_device_ptr(v)) {
+    *out = any_positive(v, n);
+    return GFR_OK;
+  }
+  unsigned int* flag = nullptr;
+  unsigned int h = 0;
+  hipError_t e = hipMalloc((void**)&flag, sizeof(unsigned int));
+  if (e == hipSuccess) e = hipMemcpy(flag, &h, sizeof(h), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_any_positive(v, n, flag, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(&h, flag, sizeof(h), hipMemcpyDeviceToHost);
+  if (flag) (void)hipFree(flag);
+  if (e != hipSuccess) return hip_err(ctx, e, "gfr_recon_create background check");
+  *out = h != 0;
+  return GFR_OK;
+}
+
 int rfail(gfr_recon* r, int code) {
   gfr_recon_destroy(r);
   return code;
@@ -198,8 +217,13 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   if (p->mode >= 2 && !t2) return set_err(ctx, GFR_EINVAL, "triples sinogram required");
   if (p->mode == 3 && nrec > 0 && !rec) return set_err(ctx, GFR_EINVAL, "VLOR records required");
   const long long nd = h.info.ndata;
-  const bool hasBgD = any_positive(d2 + nd, nd);
-  const bool hasBgT = p->mode >= 2 && any_positive(t2 + nd, nd);
+  // Background halves that are all <= 0 are not stored (bg = 0 in the ratio).
+  bool hasBgD = false, hasBgT = false;
+  {
+    int rc = bg_present(ctx, d2 + nd, nd, &hasBgD);
+    if (rc == GFR_OK && p->mode >= 2) rc = bg_present(ctx, t2 + nd, nd, &hasBgT);
+    if (rc != GFR_OK) return rc;
+  }
   const Plan pl = plan_of(h, p, hasBgD, hasBgT);
   const size_t need = plan_bytes(pl);
   {
@@ -244,7 +268,7 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   if (hipMalloc((void**)&stage, pl.sino_f32) != hipSuccess)
     return rfail(r, set_err(ctx, GFR_ENOMEM, "reconstruction staging allocation failed"));
   auto upload = [&](const float* host, void* dst) -> hipError_t {
-    hipError_t ee = hipMemcpyAsync(stage, host, pl.sino_f32, hipMemcpyHostToDevice, st);
+    hipError_t ee = hipMemcpyAsync(stage, host, pl.sino_f32, hipMemcpyDefault, st);
     if (ee == hipSuccess) ee = launch_convert_store(stage, dst, p->storage, nd, st);
     if (ee == hipSuccess) ee = hipStreamSynchronize(st);
     return ee;
@@ -255,7 +279,7 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   FwdOut fo;
   fo.proj = r->pb;
   if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fo, 0, 1, nullptr, r->tune, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(stage, norm, pl.sino_f32, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(stage, norm, pl.sino_f32, hipMemcpyDefault, st);
   if (e == hipSuccess) e = launch_calib_mask(G, r->pb, stage, r->mask, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   // Measured data, stored unscaled; kernels scale by RES on load (GFIRST.cu:224-260).
@@ -272,7 +296,7 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create"));
   if (p->mode == 3) {
     // BLANKT needs T x RES (GFIRST.cu:225-226, 246): scale a float copy.
-    e = hipMemcpyAsync(r->ptc, t2, pl.sino_f32, hipMemcpyHostToDevice, st);
+    e = hipMemcpyAsync(r->ptc, t2, pl.sino_f32, hipMemcpyDefault, st);
     if (e == hipSuccess) e = launch_scale_copy(r->ptc, (float)g->res, r->dv, nd, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create vlor"));

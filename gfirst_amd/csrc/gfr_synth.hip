@@ -93,7 +93,8 @@ __device__ long long poisson(double lam, Stream& rs) {
   return (long long)floor(lam + 0.5);
 }
 
-__global__ void k_poisson(const float* __restrict__ mean, double scale, uint64_t seed,
+__global__ void k_poisson(const float* __restrict__ mean, double scale,
+                          const float* __restrict__ mean2, double scale2, uint64_t seed,
                           float* __restrict__ out, long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -102,30 +103,17 @@ __global__ void k_poisson(const float* __restrict__ mean, double scale, uint64_t
   rs.k0 = (uint32_t)seed;
   rs.k1 = (uint32_t)(seed >> 32);
   rs.used = 4;
-  out[i] = (float)poisson((double)mean[i] * scale, rs);
-}
-
-__global__ void k_axpby(const float* __restrict__ a, float sa, const float* __restrict__ b,
-                        float sb, float* __restrict__ out, long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = sa * a[i] + (b ? sb * b[i] : 0.f);
+  const double lam = (double)mean[i] * scale + (mean2 ? (double)mean2[i] * scale2 : 0.0);
+  out[i] = (float)poisson(lam, rs);
 }
 
 }  // namespace
 
-hipError_t launch_poisson(const float* mean, double scale, uint64_t seed, float* out, long long n,
-                          hipStream_t st) {
+hipError_t launch_poisson(const float* mean, double scale, const float* mean2, double scale2,
+                          uint64_t seed, float* out, long long n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_poisson, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, mean, scale,
-                     seed, out, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_axpby(const float* a, float sa, const float* b, float sb, float* out,
-                        long long n, hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, sa, b, sb,
-                     out, n);
+                     mean2, scale2, seed, out, n);
   return hipGetLastError();
 }
 

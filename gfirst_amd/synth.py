@@ -86,3 +86,43 @@ def as_file_pair(prompts, background=None):
     if background is not None:
         out[nd:] = background
     return out
+
+
+def frame_schedule(nframes):
+    """Frame start/end times (min) of a dynamic acquisition: short frames
+    first, then longer ones (10 s, 30 s, 1 min, 2 min, 5 min blocks)."""
+    blocks = [(10 / 60.0, 12), (0.5, 12), (1.0, 12), (2.0, 14), (5.0, 14)]
+    t, out = 0.0, []
+    while len(out) < nframes:
+        for d, n in blocks:
+            for _ in range(n):
+                if len(out) < nframes:
+                    out.append((t, t + d))
+                    t += d
+        blocks = [(5.0, nframes)]
+    return np.array(out, np.float64)
+
+
+def two_compartment(t, K1, k2, A=(8.0, 1.5), lam=(2.0, 0.05)):
+    """Plasma input Cp(t) = sum_i A_i exp(-lam_i t) and tissue response of a
+    one-tissue (two-compartment) model, Ct = K1 Cp (*) exp(-k2 t), in closed
+    form: Ct(t) = K1 sum_i A_i (exp(-lam_i t) - exp(-k2 t)) / (k2 - lam_i)."""
+    t = np.asarray(t, np.float64)
+    cp = sum(a * np.exp(-l * t) for a, l in zip(A, lam))
+    ct = K1 * sum(a * (np.exp(-l * t) - np.exp(-k2 * t)) / (k2 - l) for a, l in zip(A, lam))
+    return cp, ct
+
+
+def dynamic_study(nframes, seed=0, counts=(1e5, 1e7), subsets=(8, 16, 24)):
+    """Per-frame parameters of the batched dynamic study: mid-frame time,
+    seeded subset count, log-uniform total counts and the two compartment
+    weights (plasma, tissue) of that frame."""
+    rng = np.random.default_rng(seed)
+    fr = frame_schedule(nframes)
+    tm = fr.mean(axis=1)
+    K1, k2 = rng.uniform(0.1, 0.6), rng.uniform(0.05, 0.3)
+    cp, ct = two_compartment(tm, K1, k2)
+    nsub = rng.choice(np.asarray(subsets), size=nframes)
+    cts = np.exp(rng.uniform(np.log(counts[0]), np.log(counts[1]), size=nframes))
+    return [dict(t=float(tm[k]), nsub=int(nsub[k]), counts=float(cts[k]), cp=float(cp[k]),
+                 ct=float(ct[k])) for k in range(nframes)]

@@ -239,8 +239,9 @@ long long gfr_vlor_bin_hits(const gfr_geometry* g, const float* hits, long long 
 /* osem3d (DRV:44-575) as a session: uploads the data once, keeps every
  * array resident in HBM, and advances one (type, subset) pass per step.
  * d2/t2: 2*ndata floats as read from file (unscaled); t2 NULL in mode 1;
- * norm: ndata floats; rec/nrec: VLOR records (mode 3). Admission checks the
- * context's HBM budget. */
+ * norm: ndata floats; d2/t2/norm may be host or device memory (device data
+ * complete before the call); rec/nrec: VLOR records in host memory (mode 3).
+ * Admission checks the context's HBM budget. */
 int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params* p,
                      const float* d2, const float* t2, const float* norm, const int32_t* rec,
                      long long nrec, gfr_recon** out);
@@ -329,13 +330,66 @@ int gfr_batch_stats(gfr_batch* b, long long* launches, long long* builds);
 void* gfr_batch_stream(gfr_batch* b);
 int gfr_batch_destroy(gfr_batch* b);
 
+/* ---------------- serving loop ---------------- */
+
+/* Reconstruction requests (any geometry, mode, storage and count level)
+ * served from one context: each request is admitted as a device-resident
+ * session (gfr_recon_create) when fewer than max_active are running and the
+ * context's HBM budget has room (arrival order; a request that cannot fit on
+ * an idle server fails with GFR_ENOMEM), and every admitted session advances
+ * one pass at a time on its own stream, at most `inflight` passes queued
+ * ahead of the device, so requests join and leave the running set between
+ * passes and their kernels overlap. A request's result is bit-identical to
+ * the same session run alone. Host threads inside the server drive the
+ * device; the caller only submits and collects. The reference runs one study
+ * per process (DRV:577-650). */
+typedef struct gfr_server gfr_server;
+#define GFR_REQ_PENDING 0
+#define GFR_REQ_ACTIVE 1
+#define GFR_REQ_DONE 2
+#define GFR_REQ_FAILED 3
+typedef struct gfr_request_info {
+  int state;        /* GFR_REQ_*                                    */
+  int error;        /* status code of a failed request              */
+  double t_submit;  /* seconds on the server clock                  */
+  double t_admit;   /* session created                              */
+  double t_done;    /* result on the host (or failure)              */
+  long long passes; /* passes enqueued so far                       */
+  long long bytes;  /* device bytes of the session                  */
+} gfr_request_info;
+typedef struct gfr_server_totals {
+  long long submitted, pending, active, done, failed, passes;
+  double now; /* server clock, seconds */
+} gfr_server_totals;
+int gfr_server_create(gfr_ctx* ctx, int max_active, int inflight, gfr_server** out);
+/* Queues a request; arguments as gfr_recon_create (d2/t2/norm in host or
+ * device memory, device data complete before the call). The pointed-to
+ * inputs must stay valid until the request has left GFR_REQ_PENDING. */
+int gfr_server_submit(gfr_server* s, const gfr_geometry* g, const gfr_recon_params* p,
+                      const float* d2, const float* t2, const float* norm, const int32_t* rec,
+                      long long nrec, long long* id);
+int gfr_server_info(gfr_server* s, long long id, gfr_request_info* info);
+/* Blocks until the request is done or failed (timeout_s < 0: no limit);
+ * returns its state. */
+int gfr_server_wait(gfr_server* s, long long id, double timeout_s);
+/* Copies the images of a done request (nvox floats each; xT in modes 2-3);
+ * a failed request returns its error code. */
+int gfr_server_result(gfr_server* s, long long id, float* xD, float* xT);
+/* Forgets a done or failed request (frees its host result). */
+int gfr_server_release(gfr_server* s, long long id);
+int gfr_server_stats(gfr_server* s, gfr_server_totals* st);
+/* Stops admitting, drops pending requests, waits for and frees running
+ * sessions. */
+int gfr_server_destroy(gfr_server* s);
+
 /* ---------------- synthetic measurements ---------------- */
 
-/* out[i] = Poisson(scale * mean[i]) as floats, seeded counter-based RNG
- * (Philox4x32-10): the draw of bin i depends only on (seed, i). Device
- * pointers; the reference reads measured files only. */
-int gfr_poisson(gfr_ctx* ctx, const float* d_mean, double scale, unsigned long long seed,
-                float* d_out, long long n, void* stream);
+/* out[i] = Poisson(scale * mean[i] + scale2 * mean2[i]) as floats (mean2 may
+ * be NULL), seeded counter-based RNG (Philox4x32-10): the draw of bin i
+ * depends only on (seed, i). Device pointers; the reference reads measured
+ * files only. */
+int gfr_poisson(gfr_ctx* ctx, const float* d_mean, double scale, const float* d_mean2,
+                double scale2, unsigned long long seed, float* d_out, long long n, void* stream);
 
 #ifdef __cplusplus
 }
