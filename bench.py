@@ -83,6 +83,19 @@ def alg_flops(g, nsub):
     return dict(forward=22.0 * L * (g.res + 1), back=10.0 * g.nvox * nviews)
 
 
+def measured_traffic(dom, back_zb, storage):
+    """HBM bytes per launch of the backprojector from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE runs (scaled by a plain copy's counts),
+    for the tile shape in use; None when no measurement matches."""
+    path = os.path.join(ROOT, "profiles", "k_back_c1_traffic.json")
+    if dom != "back" or storage != 0 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        m = json.load(f)
+    e = m.get("by_back_zb", {}).get(str(back_zb))
+    return None if e is None else e["fetch_bytes"] + e["write_bytes"]
+
+
 def cpu_baseline(g, d2, norm, pb, nsub, nsteps):
     """The CPU oracle (FP32, OpenMP) on a bounded sample: nsteps
     subset-iterations of the same study (the untimed mask set-up reuses the
@@ -174,7 +187,8 @@ def run_c1(args):
         "phase_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in ph.items() if v[1]},
         "roofline": {"kernel": "k_forward" if dom == "forward" else "k_back", "bound": "hbm",
                      "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 5),
+                     "traffic": measured_traffic(dom, tune[0], storage),
                      "alg_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 4),
                      "fp32_tflops": round(flops, 2),
                      "fp32_frac": round(flops / PEAK_FP32_TFLOPS, 4)},
