@@ -83,12 +83,12 @@ def alg_flops(g, nsub):
     return dict(forward=22.0 * L * (g.res + 1), back=10.0 * g.nvox * nviews)
 
 
-def measured_traffic(dom, back_zb, storage):
+def measured_traffic(dom, back_zb, storage, back_lds):
     """HBM bytes per launch of the backprojector from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE runs (scaled by a plain copy's counts),
     for the tile shape in use; None when no measurement matches."""
     path = os.path.join(ROOT, "profiles", "k_back_c1_traffic.json")
-    if dom != "back" or storage != 0 or not os.path.exists(path):
+    if dom != "back" or storage != 0 or back_lds or not os.path.exists(path):
         return None
     with open(path) as f:
         m = json.load(f)
@@ -188,13 +188,13 @@ def run_c1(args):
         "roofline": {"kernel": "k_forward" if dom == "forward" else "k_back", "bound": "hbm",
                      "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 5),
-                     "traffic": measured_traffic(dom, tune[0], storage),
+                     "traffic": measured_traffic(dom, tune[0], storage, tune[5]),
                      "alg_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 4),
                      "fp32_tflops": round(flops, 2),
                      "fp32_frac": round(flops / PEAK_FP32_TFLOPS, 4)},
         "loglik_last_pass": ll,
         "tiles": {"back_slices_per_thread": tune[0], "zsum_sinograms_per_wave": tune[1],
-                  "autotuned": tune[2]},
+                  "back_lds_window": tune[5], "autotuned": tune[2]},
     }
     r.close()
     ctx.close()

@@ -114,15 +114,17 @@ class Context:
 
     # -- tile shapes of the projectors -----------------------------------------
     def tuning(self, g, nsub):
-        """(back_zb, zsum_sg, measured, ms_back, ms_zsum) for (geometry, nsub)."""
-        zb, sg = C.c_int(), C.c_int()
+        """(back_zb, zsum_sg, measured, ms_back, ms_zsum, back_lds) for (geometry, nsub)."""
+        zb, sg, ld = C.c_int(), C.c_int(), C.c_int()
         mb, mz = C.c_float(), C.c_float()
         rc = _check(self.h, lib().gfr_tuning_get(self.h, C.byref(g.c()), nsub, C.byref(zb),
-                                                 C.byref(sg), C.byref(mb), C.byref(mz)))
-        return zb.value, sg.value, bool(rc), mb.value, mz.value
+                                                 C.byref(sg), C.byref(ld), C.byref(mb),
+                                                 C.byref(mz)))
+        return zb.value, sg.value, bool(rc), mb.value, mz.value, bool(ld.value)
 
-    def set_tuning(self, g, nsub, back_zb, zsum_sg):
-        _check(self.h, lib().gfr_tuning_set(self.h, C.byref(g.c()), nsub, back_zb, zsum_sg))
+    def set_tuning(self, g, nsub, back_zb, zsum_sg, back_lds=False):
+        _check(self.h, lib().gfr_tuning_set(self.h, C.byref(g.c()), nsub, back_zb, zsum_sg,
+                                            1 if back_lds else 0))
 
     # -- operators (one per reference wrapper) ---------------------------------
     def fill_cylinder(self, g, val, out=None):

@@ -30,6 +30,7 @@ struct DevGeom {
   float RADM, RESM;   // (NRAD-1)/2, (RES-1)/2
   float fxf;          // forward: detector pixel -> voxel (px/vx)
   float fxb;          // backward: voxel -> detector pixel (vx/px)
+  float fzb_absmax;   // largest |fzb| over the segments
   long long ndata, nvox;
   const float* sin_t;     // [nang]
   const float* cos_t;     // [nang]
@@ -47,6 +48,7 @@ struct Taps9 { float g[9]; };
 struct Tune {
   int back_zb = 8;   // slices per backprojector thread: 4, 8 or 16
   int zsum_sg = 8;   // consecutive sinograms per z-sum wave: 4, 8 or 16
+  bool back_lds = false;  // backprojector reads its rows through an LDS window
   bool tuned = false;
   float ms_back = 0.f, ms_zsum = 0.f;
 };

@@ -213,16 +213,319 @@ __global__ void k_pack(DevGeom G, const float* __restrict__ corr, const uint8_t*
 }
 
 
+// ---------------------------------------------------------------------------
+// Backprojector (BACKWPROJ_GPU, GFIRST_GPU_Kernels.cu:112-160): per voxel and
+// view, bilinear (radial bin, plane) interpolation of the packed {ratio,
+// calib} rows, the -1000 gap rule, sums over views in the reference's order
+// (segments outer, angles inner, GFIRST.cu:439-467).
+//
 // Block: 4 waves, each an 8x8 tile of voxel columns (16x16 per block) times
-// ZB slices. Across an 8x8 tile the z tilt of a view moves by a few planes
-// only, so a wave with no slice inside the view's plane range skips the view
-// (wave-uniform test); the tile's radial bins are a few neighbouring bins, so
-// row loads stay within a few cache lines.
+// ZB slices. Per view the block needs a small window of the view's rows: the
+// bins its 16x16 columns project to (<= 16*sqrt2*fxb + 2) times the planes
+// its slices tilt to (<= ZB + 16*sqrt2*|fzb| + 2). The window is copied into
+// LDS with coalesced loads, double-buffered so that the next view's window is
+// in flight while the current one is used; the per-voxel reads then come from
+// LDS. Window bounds are exact: every per-lane coordinate is a monotone
+// function (correctly rounded fma/add, floor) of the column position and the
+// slice, so its extremes over the block are at the corner columns and the
+// first/last slice.
+// ---------------------------------------------------------------------------
+constexpr int kBW = 33;  // LDS row stride in float2 (odd: rows fall on different banks)
+constexpr int kBinsMax = 32;
+
+// Per-lane state of one view.
+struct LaneView {
+  float yr, w0, wl, s0, au;
+  int r0, r1;
+  bool any;
+};
+
+__device__ __forceinline__ LaneView lane_view(const DevGeom& G, float xv, float yv, float st,
+                                              float ct, float fzb, float fz0, float fzl,
+                                              float fzm, float fzhi, bool act) {
+  LaneView v;
+  v.yr = fmaf(yv, ct, -(xv * st));
+  v.s0 = fmaf(v.yr, fzb, fz0);
+  const float sl = fmaf(v.yr, fzb, fzl);
+  v.w0 = v.s0 + 0.5f;
+  v.wl = sl + 0.5f;
+  v.any = act && v.wl >= fzm && v.w0 < fzhi;
+  const float xr = fmaf(yv, st, xv * ct);
+  const float u = fmaf(xr, G.fxb, G.RADM) + 0.5f;
+  axis(u, G.nrad, v.r0, v.r1, v.au);
+  return v;
+}
+
+// Accumulates one view into n[], d[]. F(ii, c) returns the {ratio, calib}
+// pair of plane ii (absolute) at bin column c (0: r0, 1: r1) for this lane.
+template <int ZB, class F>
+__device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, float yr, int zb,
+                                                int nzs, float fzm, float fzhi, F&& fetch,
+                                                float* n, float* d) {
+  // Shared z axis. Slice j samples plane position w_j = fl(fl(P + zb + j) + 0.5)
+  // with P = yr * fzb exact inside the fma. When s0 = fl(P + zb) >= 1 and
+  // w_{ZB-1} lie in one binade [2^E, 2^(E+1)), every exact sum P + zb + j
+  // (j >= 1) lies in that binade too and rounds on its grid (ulp <= 2^-16, so
+  // integer shifts keep the nearest point and the tie parity): fl(P + zb + j) =
+  // s0 + j, w_j = s0 + j + 0.5 and t_j = w_j - 0.5 = s0 + j exactly. Then all
+  // slices share alpha = s0 - floor(s0) and take planes floor(s0) + j, j + 1.
+  const bool slow = v.any && !(v.s0 >= 1.f && (__float_as_uint(v.s0) >> 23) ==
+                                                   (__float_as_uint(v.wl) >> 23));
+  if (__builtin_amdgcn_ballot_w64(slow) == 0) {
+    const float f0 = floorf(v.s0);
+    const float al = v.s0 - f0;
+    const int i0 = (int)f0;
+    float c[ZB + 1], q[ZB + 1];
+#pragma unroll
+    for (int k = 0; k <= ZB; k++) {
+      const float2 a0 = fetch(i0 + k, 0), a1 = fetch(i0 + k, 1);
+      c[k] = lerp1(a0.x, a1.x, v.au);
+      q[k] = lerp1(a0.y, a1.y, v.au);
+    }
+#pragma unroll
+    for (int j = 0; j < ZB; j++) {
+      const float wj = v.w0 + (float)j;  // exact, see above
+      const float A = lerp1(c[j], c[j + 1], al);
+      const float B = lerp1(q[j], q[j + 1], al);
+      // in range and not a gap (GFIRST_GPU_Kernels.cu:157)
+      const bool ok =
+          v.any && (zb + j < nzs) && wj >= fzm && wj < fzhi && !(B < 0.f) && !(A < 0.f);
+      n[j] += ok ? A : 0.f;
+      d[j] += ok ? B : 0.f;
+    }
+    return;
+  }
+  // General case: one plane position per slice, as the reference computes it.
+  float al[ZB];
+  int ii[ZB];
+  bool inr[ZB];
+#pragma unroll
+  for (int j = 0; j < ZB; j++) {
+    const float w = (j == 0) ? v.w0 : (j == ZB - 1) ? v.wl : fmaf(yr, fzb, (float)(zb + j)) + 0.5f;
+    inr[j] = v.any && (zb + j < nzs) && w >= fzm && w < fzhi;
+    const float t = w - 0.5f;
+    const float f = floorf(t);
+    al[j] = t - f;
+    ii[j] = (int)f;
+  }
+  bool reg = true;
+#pragma unroll
+  for (int j = 1; j < ZB; j++) reg = reg && (!inr[j] || ii[j] == ii[0] + j);
+  float av[ZB], bv[ZB];
+  if (reg) {
+    float c[ZB + 1], q[ZB + 1];
+#pragma unroll
+    for (int k = 0; k <= ZB; k++) {
+      const float2 a0 = fetch(ii[0] + k, 0), a1 = fetch(ii[0] + k, 1);
+      c[k] = lerp1(a0.x, a1.x, v.au);
+      q[k] = lerp1(a0.y, a1.y, v.au);
+    }
+#pragma unroll
+    for (int j = 0; j < ZB; j++) {
+      av[j] = inr[j] ? lerp1(c[j], c[j + 1], al[j]) : 0.f;
+      bv[j] = inr[j] ? lerp1(q[j], q[j + 1], al[j]) : 0.f;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < ZB; j++) {
+      av[j] = 0.f;
+      bv[j] = 0.f;
+      if (inr[j]) {
+        const float2 w00 = fetch(ii[j], 0), w01 = fetch(ii[j], 1);
+        const float2 w10 = fetch(ii[j] + 1, 0), w11 = fetch(ii[j] + 1, 1);
+        av[j] = lerp1(lerp1(w00.x, w01.x, v.au), lerp1(w10.x, w11.x, v.au), al[j]);
+        bv[j] = lerp1(lerp1(w00.y, w01.y, v.au), lerp1(w10.y, w11.y, v.au), al[j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < ZB; j++) {
+    float A = av[j], B = bv[j];
+    if (B < 0.f || A < 0.f) {  // gaps (GFIRST_GPU_Kernels.cu:157)
+      A = 0.f;
+      B = 0.f;
+    }
+    n[j] += A;
+    d[j] += B;
+  }
+}
+
+// Block-uniform window of one view: planes [lo, lo + nr) and bins
+// [rlo, rlo + nb) cover every access of the block's lanes.
+struct Window {
+  int lo, nr, rlo, nb;
+  bool used;  // some column and slice of the block can fall in the segment's planes
+};
+
+__device__ __forceinline__ Window view_window(const DevGeom& G, float x0, float x1, float y0,
+                                              float y1, float st, float ct, float fzb, float fz0,
+                                              float fzl, float fzm, float fzhi) {
+  // yr = fma(yv, ct, -(xv*st)) and xr = fma(yv, st, xv*ct) are monotone in
+  // xv and yv: extremes at the corners.
+  const float ya = fmaf(y0, ct, -(x0 * st)), yb = fmaf(y0, ct, -(x1 * st));
+  const float yc = fmaf(y1, ct, -(x0 * st)), yd = fmaf(y1, ct, -(x1 * st));
+  const float ymn = fminf(fminf(ya, yb), fminf(yc, yd));
+  const float ymx = fmaxf(fmaxf(ya, yb), fmaxf(yc, yd));
+  const float pa = fmaf(ymn, fzb, fz0), pb = fmaf(ymx, fzb, fz0);
+  const float qa = fmaf(ymn, fzb, fzl), qb = fmaf(ymx, fzb, fzl);
+  const float smn = fminf(pa, pb), smx = fmaxf(qa, qb);
+  const float wmn = smn + 0.5f, wmx = smx + 0.5f;
+  Window wd;
+  wd.used = wmx >= fzm && wmn < fzhi;
+  const int lo = (int)floorf(wmn - 0.5f);
+  const int hi = (int)floorf(wmx - 0.5f) + 1;
+  wd.lo = lo;
+  wd.nr = hi - lo + 1;
+  const float xa = fmaf(y0, st, x0 * ct), xb = fmaf(y0, st, x1 * ct);
+  const float xc = fmaf(y1, st, x0 * ct), xd = fmaf(y1, st, x1 * ct);
+  const float umn = fmaf(fminf(fminf(xa, xb), fminf(xc, xd)), G.fxb, G.RADM) + 0.5f;
+  const float umx = fmaf(fmaxf(fmaxf(xa, xb), fmaxf(xc, xd)), G.fxb, G.RADM) + 0.5f;
+  int ra, rb, rc, rd;
+  float dummy;
+  axis(umn, G.nrad, ra, rb, dummy);
+  axis(umx, G.nrad, rc, rd, dummy);
+  wd.rlo = ra;
+  wd.nb = rd - ra + 1;
+  return wd;
+}
+
 template <int ZB>
 __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restrict__ c2,
                                               unsigned c2_bytes, float* __restrict__ numer,
                                               float* __restrict__ denom, int subset, int nsub,
-                                              const int* __restrict__ dsub) {
+                                              const int* __restrict__ dsub, int rmax) {
+  extern __shared__ float2 tile[];  // [2][rmax][kBW]
+  if (dsub) subset = *dsub;
+  const int nav = (G.nang - subset + nsub - 1) / nsub;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  // blocks of one z-slab are consecutive logical ids, and each XCD walks a
+  // contiguous range of logical ids
+  const unsigned nbx = (G.res + 15) / 16, nxy = nbx * nbx;
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const int bz = (int)(b / nxy), byx = (int)(b - (unsigned)bz * nxy);
+  const int bx = byx % (int)nbx, by = byx / (int)nbx;
+  const int ix = bx * 16 + (wv & 1) * 8 + (lane & 7);
+  const int iy = by * 16 + (wv >> 1) * 8 + (lane >> 3);
+  const int zb = bz * ZB;
+  const bool act = ix < G.res && iy < G.res;
+  const __amdgpu_buffer_rsrc_t rc2 = rsrc(c2, c2_bytes);
+  const float xv = (float)ix - G.RESM, yv = (float)iy - G.RESM;
+  // active corner columns of the block
+  const float cx0 = (float)(bx * 16) - G.RESM, cx1 = (float)min(bx * 16 + 15, G.res - 1) - G.RESM;
+  const float cy0 = (float)(by * 16) - G.RESM, cy1 = (float)min(by * 16 + 15, G.res - 1) - G.RESM;
+  const unsigned rowb = (unsigned)G.nrad * 8u;
+  const unsigned nr = (unsigned)(G.nsinogs + 2 * G.nseg);
+  const float fz0 = (float)zb, fzl = (float)(zb + ZB - 1);
+  float n[ZB], d[ZB];
+#pragma unroll
+  for (int j = 0; j < ZB; j++) n[j] = d[j] = 0.f;
+  // window loader: thread t copies column t % 32 of rows t / 32 + 8 k
+  const int lc = tid & 31, lr = tid >> 5;
+  constexpr int kPre = 6;  // rows per thread: rmax <= 48
+  float2 pre[kPre];
+  int cur_si = 0, cur_a = -1;
+  Window cw{0, 0, 0, 0, false};
+  unsigned cvrow = 0;
+  // finds the next view (seg-outer, angle-inner) the block touches
+  auto next_view = [&](int& si, int& a, Window& w, unsigned& vrow) -> bool {
+    for (;;) {
+      if (++a >= nav) {
+        a = 0;
+        if (++si >= G.nseg) return false;
+      }
+      const SegEntry se = G.seg[si];
+      const int it = subset + a * nsub;
+      w = view_window(G, cx0, cx1, cy0, cy1, G.sin_t[it], G.cos_t[it], se.fzb, fz0, fzl,
+                      (float)se.zmin, (float)(G.nzs - se.zmin));
+      if (!w.used) continue;
+      vrow = (unsigned)a * nr + (unsigned)(se.nza + 2 * si + 1 - se.zmin);
+      return true;
+    }
+  };
+  // The host launches this kernel only when its row bound fits rmax and the
+  // bin bound fits kBinsMax; the clamps below keep every LDS access inside
+  // its buffer regardless.
+  auto issue = [&](const Window& w, unsigned vrow) {
+    const int nrr = min(w.nr, rmax), nbb = min(w.nb, kBinsMax);
+#pragma unroll
+    for (int k = 0; k < kPre; k++) {
+      const int r = lr + 8 * k;
+      pre[k] = make_float2(0.f, 0.f);
+      if (r < nrr && lc < nbb)
+        pre[k] = ld_f2(rc2, (vrow + (unsigned)(w.lo + r)) * rowb + (unsigned)(w.rlo + lc) * 8u, 0);
+    }
+  };
+  auto commit = [&](const Window& w, int buf) {
+    const int nrr = min(w.nr, rmax), nbb = min(w.nb, kBinsMax);
+    float2* t = tile + buf * rmax * kBW;
+#pragma unroll
+    for (int k = 0; k < kPre; k++) {
+      const int r = lr + 8 * k;
+      if (r < nrr && lc < nbb) t[r * kBW + lc] = pre[k];
+    }
+  };
+  bool have = next_view(cur_si, cur_a, cw, cvrow);
+  if (have) {
+    issue(cw, cvrow);
+    commit(cw, 0);
+  }
+  __syncthreads();
+  int buf = 0;
+  while (have) {
+    int nsi = cur_si, na = cur_a;
+    Window nw;
+    unsigned nvrow = 0;
+    const bool nhave = next_view(nsi, na, nw, nvrow);
+    if (nhave) issue(nw, nvrow);
+    const SegEntry se = G.seg[cur_si];
+    const int it = subset + cur_a * nsub;
+    const float fzm = (float)se.zmin, fzhi = (float)(G.nzs - se.zmin);
+    const LaneView v = lane_view(G, xv, yv, G.sin_t[it], G.cos_t[it], se.fzb, fz0, fzl, fzm,
+                                 fzhi, act);
+    if (__builtin_amdgcn_ballot_w64(v.any) != 0) {
+      const float2* t = tile + buf * rmax * kBW;
+      // columns of the lane's two bins and its rows, clamped into the window
+      // (only inactive columns can fall outside)
+      const int hc = min(cw.nb, kBinsMax) - 1;
+      const int c0 = min(max(v.r0 - cw.rlo, 0), hc);
+      const int c1 = min(max(v.r1 - cw.rlo, 0), hc);
+      const int lo = cw.lo, hr = min(cw.nr, rmax) - 1;
+      accumulate_view<ZB>(v, se.fzb, v.yr, zb, G.nzs, fzm, fzhi,
+                          [&](int ii, int c) -> float2 {
+                            const int r = min(max(ii - lo, 0), hr);
+                            return t[r * kBW + (c ? c1 : c0)];
+                          },
+                          n, d);
+    }
+    if (nhave) commit(nw, buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+    have = nhave;
+    cur_si = nsi;
+    cur_a = na;
+    cw = nw;
+    cvrow = nvrow;
+  }
+  if (!act) return;
+#pragma unroll
+  for (int j = 0; j < ZB; j++) {
+    const int iz = zb + j;
+    if (iz < G.nzs) {
+      const size_t iV = ((size_t)iz * G.res + iy) * G.res + ix;
+      numer[iV] = n[j];
+      denom[iV] = d[j];
+    }
+  }
+}
+
+// Same sums with every read from global memory: geometries whose window does
+// not fit the LDS bounds.
+template <int ZB>
+__global__ __launch_bounds__(256) void k_back_direct(DevGeom G, const float2* __restrict__ c2,
+                                                     unsigned c2_bytes, float* __restrict__ numer,
+                                                     float* __restrict__ denom, int subset,
+                                                     int nsub, const int* __restrict__ dsub) {
   if (dsub) subset = *dsub;
   const int nav = (G.nang - subset + nsub - 1) / nsub;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -240,87 +543,20 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
   for (int j = 0; j < ZB; j++) n[j] = d[j] = 0.f;
   for (int si = 0; si < G.nseg; si++) {
     const SegEntry se = G.seg[si];
-    const int zm = se.zmin;
-    const int zhi = G.nzs - zm;
-    const float fzm = (float)zm, fzhi = (float)zhi;
-    // row of plane p of this segment within an angle: segrow + p (planes
-    // zm-1 .. zhi are the padded rows)
-    const unsigned segrow = (unsigned)(se.nza + 2 * si + 1 - zm);
+    const float fzm = (float)se.zmin, fzhi = (float)(G.nzs - se.zmin);
+    const unsigned segrow = (unsigned)(se.nza + 2 * si + 1 - se.zmin);
     for (int a = 0; a < nav; a++) {
       const int it = subset + a * nsub;
-      const float st = G.sin_t[it], ct = G.cos_t[it];
-      const float yr = fmaf(yv, ct, -(xv * st));
-      const float w0 = fmaf(yr, se.fzb, fz0) + 0.5f;
-      const float wl = fmaf(yr, se.fzb, fzl) + 0.5f;
-      const bool any = act && wl >= fzm && w0 < fzhi;
-      if (__builtin_amdgcn_ballot_w64(any) == 0) continue;
-      const float xr = fmaf(yv, st, xv * ct);
-      const float u = fmaf(xr, G.fxb, G.RADM) + 0.5f;
-      int r0, r1;
-      float au;
-      axis(u, G.nrad, r0, r1, au);
-      const unsigned o0 = (unsigned)r0 * 8u, o1 = (unsigned)r1 * 8u;
+      const LaneView v = lane_view(G, xv, yv, G.sin_t[it], G.cos_t[it], se.fzb, fz0, fzl, fzm,
+                                   fzhi, act);
+      if (__builtin_amdgcn_ballot_w64(v.any) == 0) continue;
+      const unsigned o0 = (unsigned)v.r0 * 8u, o1 = (unsigned)v.r1 * 8u;
       const unsigned vrow = (unsigned)a * nr + segrow;
-      float al[ZB];
-      int ii[ZB];
-      bool inr[ZB];
-#pragma unroll
-      for (int j = 0; j < ZB; j++) {
-        const float w = (j == 0) ? w0 : (j == ZB - 1) ? wl : fmaf(yr, se.fzb, (float)(zb + j)) + 0.5f;
-        inr[j] = act && (zb + j < G.nzs) && w >= fzm && w < fzhi;
-        const float t = w - 0.5f;
-        const float f = floorf(t);
-        al[j] = t - f;
-        ii[j] = (int)f;
-      }
-      bool reg = true;
-#pragma unroll
-      for (int j = 1; j < ZB; j++) reg = reg && (!inr[j] || ii[j] == ii[0] + j);
-      float av[ZB], bv[ZB];
-      if (reg) {
-        // planes ii0 + k, k = 0..ZB: slice j blends planes j and j+1
-        const unsigned rb0 = (vrow + (unsigned)ii[0]) * rowb;
-        float c[ZB + 1], q[ZB + 1];
-        float2 v0[ZB + 1], v1[ZB + 1];
-#pragma unroll
-        for (int k = 0; k <= ZB; k++) {
-          v0[k] = ld_f2(rc2, rb0 + o0, (unsigned)k * rowb);
-          v1[k] = ld_f2(rc2, rb0 + o1, (unsigned)k * rowb);
-        }
-#pragma unroll
-        for (int k = 0; k <= ZB; k++) {
-          c[k] = lerp1(v0[k].x, v1[k].x, au);
-          q[k] = lerp1(v0[k].y, v1[k].y, au);
-        }
-#pragma unroll
-        for (int j = 0; j < ZB; j++) {
-          av[j] = inr[j] ? lerp1(c[j], c[j + 1], al[j]) : 0.f;
-          bv[j] = inr[j] ? lerp1(q[j], q[j + 1], al[j]) : 0.f;
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < ZB; j++) {
-          av[j] = 0.f;
-          bv[j] = 0.f;
-          if (inr[j]) {
-            const unsigned ra = (vrow + (unsigned)ii[j]) * rowb;
-            const float2 w00 = ld_f2(rc2, ra + o0, 0), w01 = ld_f2(rc2, ra + o1, 0);
-            const float2 w10 = ld_f2(rc2, ra + o0, rowb), w11 = ld_f2(rc2, ra + o1, rowb);
-            av[j] = lerp1(lerp1(w00.x, w01.x, au), lerp1(w10.x, w11.x, au), al[j]);
-            bv[j] = lerp1(lerp1(w00.y, w01.y, au), lerp1(w10.y, w11.y, au), al[j]);
-          }
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < ZB; j++) {
-        float A = av[j], B = bv[j];
-        if (B < 0.f || A < 0.f) {  // gaps (GFIRST_GPU_Kernels.cu:157)
-          A = 0.f;
-          B = 0.f;
-        }
-        n[j] += A;
-        d[j] += B;
-      }
+      accumulate_view<ZB>(v, se.fzb, v.yr, zb, G.nzs, fzm, fzhi,
+                          [&](int ii, int c) -> float2 {
+                            return ld_f2(rc2, (vrow + (unsigned)ii) * rowb + (c ? o1 : o0), 0);
+                          },
+                          n, d);
     }
   }
   if (!act) return;
@@ -442,6 +678,15 @@ hipError_t launch_llh_sum(const DevGeom& G, const double* part, int subset, int 
   return hipGetLastError();
 }
 
+// Rows of the backprojector's LDS window: ZB slices + 1 for the second
+// interpolation plane, the tilt of a 16x16 column tile (15*sqrt2 columns
+// times the largest |fzb|) and 2 for rounding; at most 48 (six rows per
+// loading thread), else 0 and the direct kernel reads global memory.
+int back_window_rows(const DevGeom& G, int zb) {
+  const int r = zb + 3 + (int)ceilf(21.3f * G.fzb_absmax);
+  return r <= 48 ? r : 0;  // 0: does not fit, read global memory directly
+}
+
 size_t packed_bytes(const DevGeom& G, int subset, int nsub) {
   return sizeof(float2) * (size_t)nav_of(G, subset, nsub) * (G.nsinogs + 2 * (size_t)G.nseg) *
          G.nrad;
@@ -459,7 +704,20 @@ hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* nume
                               hipStream_t st) {
   const size_t bytes = packed_bytes(G, dsub ? 0 : subset, nsub);
   if (bytes >= 0xFFFFFFF0u) return hipErrorInvalidValue;
-#define GFR_BK(ZB)                                                                                hipLaunchKernelGGL(k_back<ZB>, dim3((G.res + 15) / 16, (G.res + 15) / 16, (G.nzs + ZB - 1) / ZB),                      dim3(256), 0, st, G, corr2, (unsigned)bytes, numer, denom, subset, nsub, dsub)
+  const unsigned nbx = (G.res + 15) / 16;
+  const bool bins_fit = 21.3f * G.fxb + 3.f <= (float)kBinsMax;
+#define GFR_BK(ZB)                                                                              \
+  {                                                                                             \
+    const int rmax = back_window_rows(G, ZB);                                                   \
+    if (tune.back_lds && bins_fit && rmax > 0) {                                                \
+      const unsigned nb = nbx * nbx * (unsigned)((G.nzs + ZB - 1) / ZB);                        \
+      hipLaunchKernelGGL(k_back<ZB>, dim3(nb), dim3(256), sizeof(float2) * 2 * rmax * kBW, st,  \
+                         G, corr2, (unsigned)bytes, numer, denom, subset, nsub, dsub, rmax);    \
+    } else {                                                                                    \
+      hipLaunchKernelGGL(k_back_direct<ZB>, dim3(nbx, nbx, (G.nzs + ZB - 1) / ZB), dim3(256), 0, \
+                         st, G, corr2, (unsigned)bytes, numer, denom, subset, nsub, dsub);      \
+    }                                                                                           \
+  }
   switch (tune.back_zb) {
     case 4: GFR_BK(4); break;
     case 16: GFR_BK(16); break;
@@ -512,15 +770,19 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
       t.zsum_sg = c;
     }
   }
-  for (int c : cand) {
-    Tune tt = t;
-    tt.back_zb = c;
-    const float ms = time_it([&] {
-      return launch_back_packed(G, corr2, num, den, 0, nsub, nullptr, tt, st);
-    });
-    if (ms < bb) {
-      bb = ms;
-      t.back_zb = c;
+  for (int lds = 0; lds < 2; lds++) {
+    for (int c : cand) {
+      Tune tt = t;
+      tt.back_zb = c;
+      tt.back_lds = lds != 0;
+      const float ms = time_it([&] {
+        return launch_back_packed(G, corr2, num, den, 0, nsub, nullptr, tt, st);
+      });
+      if (ms < bb) {
+        bb = ms;
+        t.back_zb = c;
+        t.back_lds = lds != 0;
+      }
     }
   }
   (void)hipEventDestroy(e0);

@@ -124,15 +124,18 @@ int gfr_memcpy_d2h(gfr_ctx* ctx, void* dst, const void* src, size_t bytes, void*
 int gfr_zero(gfr_ctx* ctx, void* dptr, size_t bytes, void* stream);
 
 /* Tile shapes of the projectors for (geometry, nsub): back_zb = slices per
- * backprojector thread (4, 8, 16), zsum_sg = sinograms per forward z-sum wave
- * (4, 8, 16). A reconstruction session measures them once per context (env
- * GFR_AUTOTUNE=0 keeps the defaults 8/8); per-operation calls use the
- * context's current choice. Results do not depend on the shapes. No
- * counterpart in the reference (its launch shapes are fixed, GPU:105-107,
- * 166-171). get returns 1 if the shapes were measured, 0 if defaults. */
+ * backprojector thread (4, 8, 16), back_lds = 1 when the backprojector stages
+ * each view's rows for a 16x16-column tile in LDS (0: direct cached reads),
+ * zsum_sg = sinograms per forward z-sum wave (4, 8, 16). A reconstruction
+ * session measures them once per context (env GFR_AUTOTUNE=0 keeps the
+ * defaults 8/8/direct); per-operation calls use the context's current choice.
+ * Results do not depend on the shapes. No counterpart in the reference (its
+ * launch shapes are fixed, GPU:105-107, 166-171). get returns 1 if the shapes
+ * were measured, 0 if defaults. */
 int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int* back_zb, int* zsum_sg,
-                   float* ms_back, float* ms_zsum);
-int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int back_zb, int zsum_sg);
+                   int* back_lds, float* ms_back, float* ms_zsum);
+int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int back_zb, int zsum_sg,
+                   int back_lds);
 
 /* ---------------- per-operation entry points (stream ordered) ---------------- */
 

@@ -128,14 +128,21 @@ def test_back_project_c1_subset_vs_oracle(gctx, orc):
     ne, de = orc.back(g, corr, cal, 5, 80)
     close(n, ne, rtol=1e-6, atol=1e-5, what="numer c1")
     close(d, de, rtol=1e-6, atol=1e-5, what="denom c1")
+    # the LDS-window backprojector on the same views (windows at full tilt)
+    gctx.set_tuning(g, 80, 8, 8, True)
+    n2, d2 = gctx.back_project(g, corr, valid.astype(np.uint8), 5, 80)
+    close(n2, ne, rtol=0, what="numer c1 lds")
+    close(d2, de, rtol=0, what="denom c1 lds")
 
 
-@pytest.mark.parametrize("zb,sg", [(4, 4), (8, 16), (16, 8)])
-def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg):
+@pytest.mark.parametrize("zb,sg,lds", [(4, 4, False), (8, 16, False), (16, 8, False),
+                                       (4, 8, True), (8, 4, True), (16, 16, True)])
+def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg, lds):
     """Every autotuner candidate gives the oracle's bits (forward and back)."""
     g = C0
-    gctx.set_tuning(g, 4, zb, sg)
-    assert gctx.tuning(g, 4)[:3] == (zb, sg, True)
+    gctx.set_tuning(g, 4, zb, sg, lds)
+    t = gctx.tuning(g, 4)
+    assert t[:3] == (zb, sg, True) and t[5] == lds
     x = rand_img(g, 30)
     close(gctx.forward_project(g, x, 2, 4), orc.forward(g, x, 2, 4), what="fwd sg=%d" % sg)
     rng = np.random.default_rng(31)
