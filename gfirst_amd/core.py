@@ -299,16 +299,23 @@ class Recon:
     def __init__(self, ctx, g, params, d2, t2, norm, records=None):
         self.ctx, self.g, self.params = ctx, g, params
         nd = g.ndata
-        d2 = np.ascontiguousarray(d2, np.float32).reshape(-1)
-        norm = np.ascontiguousarray(norm, np.float32).reshape(-1)
-        if d2.size != 2 * nd or norm.size < nd:
-            raise ValueError("d2 must hold 2*NDATA and norm NDATA floats")
+
+        def ptr(a, n, what):
+            """Host (numpy) or device (DeviceArray) input of at least n floats."""
+            if isinstance(a, DeviceArray):
+                if a.n < n:
+                    raise ValueError("%s must hold %d floats" % (what, n))
+                return a, a.ptr
+            a = np.ascontiguousarray(a, np.float32).reshape(-1)
+            if a.size < n:
+                raise ValueError("%s must hold %d floats" % (what, n))
+            return a, a.ctypes.data
+
+        d2, d2p = ptr(d2, 2 * nd, "d2")
+        norm, normp = ptr(norm, nd, "norm")
         t2p = None
         if t2 is not None:
-            t2 = np.ascontiguousarray(t2, np.float32).reshape(-1)
-            if t2.size != 2 * nd:
-                raise ValueError("t2 must hold 2*NDATA floats")
-            t2p = t2.ctypes.data
+            t2, t2p = ptr(t2, 2 * nd, "t2")
         rec = None
         nrec = 0
         if records is not None:
@@ -316,7 +323,7 @@ class Recon:
             nrec = len(rec) // 2
         h = C.c_void_p()
         _check(ctx.h, lib().gfr_recon_create(ctx.h, C.byref(g.c()), C.byref(params),
-                                             d2.ctypes.data, t2p, norm.ctypes.data,
+                                             d2p, t2p, normp,
                                              rec.ctypes.data if rec is not None else None, nrec,
                                              C.byref(h)))
         self.h = h.value

@@ -256,11 +256,12 @@ __device__ __forceinline__ LaneView lane_view(const DevGeom& G, float xv, float 
   return v;
 }
 
-// Accumulates one view into n[], d[]. F(ii, c) returns the {ratio, calib}
-// pair of plane ii (absolute) at bin column c (0: r0, 1: r1) for this lane.
+// Accumulates one view into n[], d[]. The fetcher returns {ratio, calib}
+// pairs of this lane's two bin columns (c = 0: r0, 1: r1): rows(i0) selects
+// planes i0 .. i0 + ZB, at(k, c) reads plane i0 + k, get(ii, c) any plane.
 template <int ZB, class F>
 __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, float yr, int zb,
-                                                int nzs, float fzm, float fzhi, F&& fetch,
+                                                int nzs, float fzm, float fzhi, F& fetch,
                                                 float* n, float* d) {
   // Shared z axis. Slice j samples plane position w_j = fl(fl(P + zb + j) + 0.5)
   // with P = yr * fzb exact inside the fma. When s0 = fl(P + zb) >= 1 and
@@ -276,9 +277,10 @@ __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, fl
     const float al = v.s0 - f0;
     const int i0 = (int)f0;
     float c[ZB + 1], q[ZB + 1];
+    fetch.rows(i0);
 #pragma unroll
     for (int k = 0; k <= ZB; k++) {
-      const float2 a0 = fetch(i0 + k, 0), a1 = fetch(i0 + k, 1);
+      const float2 a0 = fetch.at(k, 0), a1 = fetch.at(k, 1);
       c[k] = lerp1(a0.x, a1.x, v.au);
       q[k] = lerp1(a0.y, a1.y, v.au);
     }
@@ -314,9 +316,10 @@ __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, fl
   float av[ZB], bv[ZB];
   if (reg) {
     float c[ZB + 1], q[ZB + 1];
+    fetch.rows(ii[0]);
 #pragma unroll
     for (int k = 0; k <= ZB; k++) {
-      const float2 a0 = fetch(ii[0] + k, 0), a1 = fetch(ii[0] + k, 1);
+      const float2 a0 = fetch.at(k, 0), a1 = fetch.at(k, 1);
       c[k] = lerp1(a0.x, a1.x, v.au);
       q[k] = lerp1(a0.y, a1.y, v.au);
     }
@@ -331,8 +334,8 @@ __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, fl
       av[j] = 0.f;
       bv[j] = 0.f;
       if (inr[j]) {
-        const float2 w00 = fetch(ii[j], 0), w01 = fetch(ii[j], 1);
-        const float2 w10 = fetch(ii[j] + 1, 0), w11 = fetch(ii[j] + 1, 1);
+        const float2 w00 = fetch.get(ii[j], 0), w01 = fetch.get(ii[j], 1);
+        const float2 w10 = fetch.get(ii[j] + 1, 0), w11 = fetch.get(ii[j] + 1, 1);
         av[j] = lerp1(lerp1(w00.x, w01.x, v.au), lerp1(w10.x, w11.x, v.au), al[j]);
         bv[j] = lerp1(lerp1(w00.y, w01.y, v.au), lerp1(w10.y, w11.y, v.au), al[j]);
       }
@@ -349,6 +352,38 @@ __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, fl
     d[j] += B;
   }
 }
+
+// Rows straight from the packed array (buffer loads, 32-bit offsets).
+struct FetchGlobal {
+  __amdgpu_buffer_rsrc_t rc;
+  unsigned vrow, rowb, o0, o1, rb0 = 0;
+  __device__ __forceinline__ void rows(int i0) { rb0 = (vrow + (unsigned)i0) * rowb; }
+  __device__ __forceinline__ float2 at(int k, int c) const {
+    return ld_f2(rc, rb0 + (c ? o1 : o0), (unsigned)k * rowb);
+  }
+  __device__ __forceinline__ float2 get(int ii, int c) const {
+    return ld_f2(rc, (vrow + (unsigned)ii) * rowb + (c ? o1 : o0), 0);
+  }
+};
+
+// Rows from the block's LDS window (row stride kBW); every index is clamped
+// into the window, which only changes lanes outside the view.
+template <int ZB>
+struct FetchLds {
+  const float2* t;
+  int lo, hr, c0, c1;
+  const float2 *p0 = nullptr, *p1 = nullptr;
+  __device__ __forceinline__ void rows(int i0) {
+    const int r = min(max(i0 - lo, 0), hr - ZB);
+    p0 = t + r * kBW + c0;
+    p1 = t + r * kBW + c1;
+  }
+  __device__ __forceinline__ float2 at(int k, int c) const { return (c ? p1 : p0)[k * kBW]; }
+  __device__ __forceinline__ float2 get(int ii, int c) const {
+    const int r = min(max(ii - lo, 0), hr);
+    return t[r * kBW + (c ? c1 : c0)];
+  }
+};
 
 // Block-uniform window of one view: planes [lo, lo + nr) and bins
 // [rlo, rlo + nb) cover every access of the block's lanes.
@@ -395,6 +430,9 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
                                               float* __restrict__ denom, int subset, int nsub,
                                               const int* __restrict__ dsub, int rmax) {
   extern __shared__ float2 tile[];  // [2][rmax][kBW]
+  __shared__ int4 wtab[256];        // windows of a chunk of 256 views
+  __shared__ short wlist[256];      // chunk entries the block touches, in view order
+  __shared__ int wcnt[4];
   if (dsub) subset = *dsub;
   const int nav = (G.nang - subset + nsub - 1) / nsub;
   const int tid = threadIdx.x;
@@ -415,7 +453,7 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
   const float cx0 = (float)(bx * 16) - G.RESM, cx1 = (float)min(bx * 16 + 15, G.res - 1) - G.RESM;
   const float cy0 = (float)(by * 16) - G.RESM, cy1 = (float)min(by * 16 + 15, G.res - 1) - G.RESM;
   const unsigned rowb = (unsigned)G.nrad * 8u;
-  const unsigned nr = (unsigned)(G.nsinogs + 2 * G.nseg);
+  const unsigned nrows = (unsigned)(G.nsinogs + 2 * G.nseg);
   const float fz0 = (float)zb, fzl = (float)(zb + ZB - 1);
   float n[ZB], d[ZB];
 #pragma unroll
@@ -424,40 +462,24 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
   const int lc = tid & 31, lr = tid >> 5;
   constexpr int kPre = 6;  // rows per thread: rmax <= 48
   float2 pre[kPre];
-  int cur_si = 0, cur_a = -1;
-  Window cw{0, 0, 0, 0, false};
-  unsigned cvrow = 0;
-  // finds the next view (seg-outer, angle-inner) the block touches
-  auto next_view = [&](int& si, int& a, Window& w, unsigned& vrow) -> bool {
-    for (;;) {
-      if (++a >= nav) {
-        a = 0;
-        if (++si >= G.nseg) return false;
-      }
-      const SegEntry se = G.seg[si];
-      const int it = subset + a * nsub;
-      w = view_window(G, cx0, cx1, cy0, cy1, G.sin_t[it], G.cos_t[it], se.fzb, fz0, fzl,
-                      (float)se.zmin, (float)(G.nzs - se.zmin));
-      if (!w.used) continue;
-      vrow = (unsigned)a * nr + (unsigned)(se.nza + 2 * si + 1 - se.zmin);
-      return true;
-    }
-  };
   // The host launches this kernel only when its row bound fits rmax and the
-  // bin bound fits kBinsMax; the clamps below keep every LDS access inside
-  // its buffer regardless.
-  auto issue = [&](const Window& w, unsigned vrow) {
-    const int nrr = min(w.nr, rmax), nbb = min(w.nb, kBinsMax);
+  // bin bound fits kBinsMax; the clamps keep every LDS access inside its
+  // buffer regardless.
+  auto issue = [&](const int4 w) {
+    const int lo = w.x, nrr = min(w.y & 0xffff, rmax), nbb = min(w.y >> 16, kBinsMax);
+    const int si = w.w & 0xffff, a = w.w >> 16;
+    const SegEntry se = G.seg[si];
+    const unsigned vrow = (unsigned)a * nrows + (unsigned)(se.nza + 2 * si + 1 - se.zmin);
 #pragma unroll
     for (int k = 0; k < kPre; k++) {
       const int r = lr + 8 * k;
       pre[k] = make_float2(0.f, 0.f);
       if (r < nrr && lc < nbb)
-        pre[k] = ld_f2(rc2, (vrow + (unsigned)(w.lo + r)) * rowb + (unsigned)(w.rlo + lc) * 8u, 0);
+        pre[k] = ld_f2(rc2, (vrow + (unsigned)(lo + r)) * rowb + (unsigned)(w.z + lc) * 8u, 0);
     }
   };
-  auto commit = [&](const Window& w, int buf) {
-    const int nrr = min(w.nr, rmax), nbb = min(w.nb, kBinsMax);
+  auto commit = [&](const int4 w, int buf) {
+    const int nrr = min(w.y & 0xffff, rmax), nbb = min(w.y >> 16, kBinsMax);
     float2* t = tile + buf * rmax * kBW;
 #pragma unroll
     for (int k = 0; k < kPre; k++) {
@@ -465,47 +487,70 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
       if (r < nrr && lc < nbb) t[r * kBW + lc] = pre[k];
     }
   };
-  bool have = next_view(cur_si, cur_a, cw, cvrow);
-  if (have) {
-    issue(cw, cvrow);
-    commit(cw, 0);
-  }
-  __syncthreads();
-  int buf = 0;
-  while (have) {
-    int nsi = cur_si, na = cur_a;
-    Window nw;
-    unsigned nvrow = 0;
-    const bool nhave = next_view(nsi, na, nw, nvrow);
-    if (nhave) issue(nw, nvrow);
-    const SegEntry se = G.seg[cur_si];
-    const int it = subset + cur_a * nsub;
-    const float fzm = (float)se.zmin, fzhi = (float)(G.nzs - se.zmin);
-    const LaneView v = lane_view(G, xv, yv, G.sin_t[it], G.cos_t[it], se.fzb, fz0, fzl, fzm,
-                                 fzhi, act);
-    if (__builtin_amdgcn_ballot_w64(v.any) != 0) {
-      const float2* t = tile + buf * rmax * kBW;
-      // columns of the lane's two bins and its rows, clamped into the window
-      // (only inactive columns can fall outside)
-      const int hc = min(cw.nb, kBinsMax) - 1;
-      const int c0 = min(max(v.r0 - cw.rlo, 0), hc);
-      const int c1 = min(max(v.r1 - cw.rlo, 0), hc);
-      const int lo = cw.lo, hr = min(cw.nr, rmax) - 1;
-      accumulate_view<ZB>(v, se.fzb, v.yr, zb, G.nzs, fzm, fzhi,
-                          [&](int ii, int c) -> float2 {
-                            const int r = min(max(ii - lo, 0), hr);
-                            return t[r * kBW + (c ? c1 : c0)];
-                          },
-                          n, d);
+  const int nviews = G.nseg * nav;  // view v: segment v / nav, angle v % nav
+  for (int c0 = 0; c0 < nviews; c0 += 256) {
+    __syncthreads();  // the previous chunk's tables and tiles are no longer read
+    // windows of this chunk's views, one per thread, and the list of the
+    // ones the block touches (ballot compaction keeps view order)
+    bool used = false;
+    {
+      const int vi = c0 + tid;
+      if (vi < nviews) {
+        const int si = vi / nav, a = vi - si * nav;
+        const SegEntry se = G.seg[si];
+        const int it = subset + a * nsub;
+        const Window w = view_window(G, cx0, cx1, cy0, cy1, G.sin_t[it], G.cos_t[it], se.fzb,
+                                     fz0, fzl, (float)se.zmin, (float)(G.nzs - se.zmin));
+        used = w.used;
+        wtab[tid] = make_int4(w.lo, (w.nr & 0xffff) | (w.nb << 16), w.rlo, si | (a << 16));
+      }
     }
-    if (nhave) commit(nw, buf ^ 1);
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(used);
+    if (lane == 0) wcnt[wv] = __popcll(m);
     __syncthreads();
-    buf ^= 1;
-    have = nhave;
-    cur_si = nsi;
-    cur_a = na;
-    cw = nw;
-    cvrow = nvrow;
+    int off = 0;
+    for (int k = 0; k < wv; k++) off += wcnt[k];
+    const int total = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    if (used) {
+      const unsigned lo32 = (unsigned)m, hi32 = (unsigned)(m >> 32);
+      const int below = __builtin_amdgcn_mbcnt_hi(hi32, __builtin_amdgcn_mbcnt_lo(lo32, 0u));
+      wlist[off + below] = (short)tid;
+    }
+    __syncthreads();
+    if (total == 0) continue;
+    int4 cw = wtab[wlist[0]];
+    issue(cw);
+    commit(cw, 0);
+    __syncthreads();
+    int buf = 0;
+    for (int e = 0; e < total; e++) {
+      const bool more = e + 1 < total;
+      int4 nw = make_int4(0, 0, 0, 0);
+      if (more) {
+        nw = wtab[wlist[e + 1]];
+        issue(nw);
+      }
+      const int si = cw.w & 0xffff, a = cw.w >> 16;
+      const SegEntry se = G.seg[si];
+      const int it = subset + a * nsub;
+      const float fzm = (float)se.zmin, fzhi = (float)(G.nzs - se.zmin);
+      const LaneView v = lane_view(G, xv, yv, G.sin_t[it], G.cos_t[it], se.fzb, fz0, fzl, fzm,
+                                   fzhi, act);
+      if (__builtin_amdgcn_ballot_w64(v.any) != 0) {
+        const int hc = min(cw.y >> 16, kBinsMax) - 1;
+        FetchLds<ZB> f;
+        f.t = tile + buf * rmax * kBW;
+        f.lo = cw.x;
+        f.hr = min(cw.y & 0xffff, rmax) - 1;
+        f.c0 = min(max(v.r0 - cw.z, 0), hc);
+        f.c1 = min(max(v.r1 - cw.z, 0), hc);
+        accumulate_view<ZB>(v, se.fzb, v.yr, zb, G.nzs, fzm, fzhi, f, n, d);
+      }
+      if (more) commit(nw, buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+      cw = nw;
+    }
   }
   if (!act) return;
 #pragma unroll
@@ -550,13 +595,13 @@ __global__ __launch_bounds__(256) void k_back_direct(DevGeom G, const float2* __
       const LaneView v = lane_view(G, xv, yv, G.sin_t[it], G.cos_t[it], se.fzb, fz0, fzl, fzm,
                                    fzhi, act);
       if (__builtin_amdgcn_ballot_w64(v.any) == 0) continue;
-      const unsigned o0 = (unsigned)v.r0 * 8u, o1 = (unsigned)v.r1 * 8u;
-      const unsigned vrow = (unsigned)a * nr + segrow;
-      accumulate_view<ZB>(v, se.fzb, v.yr, zb, G.nzs, fzm, fzhi,
-                          [&](int ii, int c) -> float2 {
-                            return ld_f2(rc2, (vrow + (unsigned)ii) * rowb + (c ? o1 : o0), 0);
-                          },
-                          n, d);
+      FetchGlobal f;
+      f.rc = rc2;
+      f.vrow = (unsigned)a * nr + segrow;
+      f.rowb = rowb;
+      f.o0 = (unsigned)v.r0 * 8u;
+      f.o1 = (unsigned)v.r1 * 8u;
+      accumulate_view<ZB>(v, se.fzb, v.yr, zb, G.nzs, fzm, fzhi, f, n, d);
     }
   }
   if (!act) return;
