@@ -88,11 +88,12 @@ def measured_traffic(dom, back_zb, storage, back_lds):
     rocprofv3 FETCH_SIZE/WRITE_SIZE runs (scaled by a plain copy's counts),
     for the tile shape in use; None when no measurement matches."""
     path = os.path.join(ROOT, "profiles", "k_back_c1_traffic.json")
-    if dom != "back" or storage != 0 or back_lds or not os.path.exists(path):
+    if dom != "back" or storage != 0 or not os.path.exists(path):
         return None
     with open(path) as f:
         m = json.load(f)
-    e = m.get("by_back_zb", {}).get(str(back_zb))
+    key = ("lds:" if back_lds else "direct:") + str(back_zb)
+    e = m.get("by_kind_slices", {}).get(key)
     return None if e is None else e["fetch_bytes"] + e["write_bytes"]
 
 

@@ -58,7 +58,8 @@ Sizes sizes_of(const GeomHost& h, const gfr_batch_params* p) {
   Sizes z{};
   const size_t nav = (size_t)(h.g.nang + p->min_nsub - 1) / p->min_nsub;
   z.img = sizeof(float) * (size_t)h.info.nvox;
-  z.corr2 = sizeof(float2) * nav * ((size_t)h.info.nsinogs + 2 * (size_t)h.g.nseg) * h.g.nrad;
+  z.corr2 = sizeof(float2) * nav * ((size_t)h.info.nsinogs + 2 * (size_t)h.g.nseg) *
+            ((size_t)h.g.nrad + 2);  // packed rows: NRAD + 2 entries
   z.rot = sizeof(float) * (size_t)(h.info.lambda1 - h.info.lambda0 + 1) * h.g.nzs * h.g.nrad *
           std::min(nav, (size_t)kRotChunk);
   z.llh = sizeof(double) * (size_t)llh_capacity(h, p->min_nsub) + 64;

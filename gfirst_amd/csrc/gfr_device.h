@@ -59,6 +59,11 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
 //    reference stores outside the segment's planes (GFIRST.cu:447);
 //  - the direct segment (zmin = 0): the extra rows repeat planes 0 and NZS-1,
 //    which is what the clamp-to-edge filter reads for planes -1 and NZS.
+// Each row holds NRAD + 2 entries: bin r at r + 1, and copies of bins 0 and
+// NRAD-1 at both ends, so the pair of bins a radial position blends under
+// clamp-to-edge, (clamp(i), clamp(i + 1)), is the two adjacent entries at
+// clamp(i, -1, NRAD-1) + 1: one 16-byte load.
+__host__ __device__ __forceinline__ size_t packed_row_len(int nrad) { return (size_t)nrad + 2; }
 __device__ __forceinline__ size_t packed_rows_per_angle(const DevGeom& G) {
   return (size_t)G.nsinogs + 2 * (size_t)G.nseg;
 }
@@ -70,9 +75,16 @@ __device__ __forceinline__ void store_packed(const DevGeom& G, float2* c2, int a
   const int k = sp.y - se.zmin;
   const int nz = G.nzs - 2 * se.zmin;
   const float2 edge = se.zmin == 0 ? v : make_float2(-1000.f, -1000.f);
-  c2[(row0 + 1 + k) * G.nrad + r] = v;
-  if (k == 0) c2[row0 * G.nrad + r] = edge;
-  if (k == nz - 1) c2[(row0 + nz + 1) * G.nrad + r] = edge;
+  const size_t L = packed_row_len(G.nrad);
+  auto put = [&](size_t row, float2 val) {
+    float2* p = c2 + row * L + r + 1;
+    p[0] = val;
+    if (r == 0) p[-1] = val;
+    if (r == G.nrad - 1) p[1] = val;
+  };
+  put(row0 + 1 + k, v);
+  if (k == 0) put(row0, edge);
+  if (k == nz - 1) put(row0 + nz + 1, edge);
 }
 
 }  // namespace gfr

@@ -33,11 +33,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, unsigned b
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
                                            0x00020000);
 }
-__device__ __forceinline__ float ld_f32(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
 __device__ __forceinline__ float2 ld_f2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
 // ---------------------------------------------------------------------------
@@ -236,7 +236,8 @@ constexpr int kBinsMax = 32;
 // Per-lane state of one view.
 struct LaneView {
   float yr, w0, wl, s0, au;
-  int r0, r1;
+  int r0, r1;  // clamped bins (GFIRST_GPU_Kernels.cu texture clamp)
+  int iq;      // clamp(floor(u - 0.5), -1, NRAD-1): packed entry pair at iq + 1
   bool any;
 };
 
@@ -252,13 +253,20 @@ __device__ __forceinline__ LaneView lane_view(const DevGeom& G, float xv, float 
   v.any = act && v.wl >= fzm && v.w0 < fzhi;
   const float xr = fmaf(yv, st, xv * ct);
   const float u = fmaf(xr, G.fxb, G.RADM) + 0.5f;
-  axis(u, G.nrad, v.r0, v.r1, v.au);
+  const float t = u - 0.5f;
+  const float f = floorf(t);
+  const int i = (int)f;
+  v.au = t - f;
+  v.r0 = min(max(i, 0), G.nrad - 1);
+  v.r1 = min(max(i + 1, 0), G.nrad - 1);
+  v.iq = min(max(i, -1), G.nrad - 1);
   return v;
 }
 
-// Accumulates one view into n[], d[]. The fetcher returns {ratio, calib}
-// pairs of this lane's two bin columns (c = 0: r0, 1: r1): rows(i0) selects
-// planes i0 .. i0 + ZB, at(k, c) reads plane i0 + k, get(ii, c) any plane.
+// Accumulates one view into n[], d[]. The fetcher returns the {ratio, calib}
+// entries of this lane's two bins (r0, r1) of a plane: rows(i0) selects
+// planes i0 .. i0 + ZB, pair(k, a0, a1) reads plane i0 + k, get2(ii, a0, a1)
+// any plane.
 template <int ZB, class F>
 __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, float yr, int zb,
                                                 int nzs, float fzm, float fzhi, F& fetch,
@@ -280,7 +288,8 @@ __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, fl
     fetch.rows(i0);
 #pragma unroll
     for (int k = 0; k <= ZB; k++) {
-      const float2 a0 = fetch.at(k, 0), a1 = fetch.at(k, 1);
+      float2 a0, a1;
+      fetch.pair(k, a0, a1);
       c[k] = lerp1(a0.x, a1.x, v.au);
       q[k] = lerp1(a0.y, a1.y, v.au);
     }
@@ -319,7 +328,8 @@ __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, fl
     fetch.rows(ii[0]);
 #pragma unroll
     for (int k = 0; k <= ZB; k++) {
-      const float2 a0 = fetch.at(k, 0), a1 = fetch.at(k, 1);
+      float2 a0, a1;
+      fetch.pair(k, a0, a1);
       c[k] = lerp1(a0.x, a1.x, v.au);
       q[k] = lerp1(a0.y, a1.y, v.au);
     }
@@ -334,8 +344,9 @@ __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, fl
       av[j] = 0.f;
       bv[j] = 0.f;
       if (inr[j]) {
-        const float2 w00 = fetch.get(ii[j], 0), w01 = fetch.get(ii[j], 1);
-        const float2 w10 = fetch.get(ii[j] + 1, 0), w11 = fetch.get(ii[j] + 1, 1);
+        float2 w00, w01, w10, w11;
+        fetch.get2(ii[j], w00, w01);
+        fetch.get2(ii[j] + 1, w10, w11);
         av[j] = lerp1(lerp1(w00.x, w01.x, v.au), lerp1(w10.x, w11.x, v.au), al[j]);
         bv[j] = lerp1(lerp1(w00.y, w01.y, v.au), lerp1(w10.y, w11.y, v.au), al[j]);
       }
@@ -353,21 +364,27 @@ __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, fl
   }
 }
 
-// Rows straight from the packed array (buffer loads, 32-bit offsets).
+// Rows straight from the packed array: one 16-byte buffer load per plane
+// gives both bins (32-bit offsets).
 struct FetchGlobal {
   __amdgpu_buffer_rsrc_t rc;
-  unsigned vrow, rowb, o0, o1, rb0 = 0;
+  unsigned vrow, rowb, oq, rb0 = 0;
   __device__ __forceinline__ void rows(int i0) { rb0 = (vrow + (unsigned)i0) * rowb; }
-  __device__ __forceinline__ float2 at(int k, int c) const {
-    return ld_f2(rc, rb0 + (c ? o1 : o0), (unsigned)k * rowb);
+  __device__ __forceinline__ void pair(int k, float2& a0, float2& a1) const {
+    const float4 q = ld_f4(rc, rb0 + oq, (unsigned)k * rowb);
+    a0 = make_float2(q.x, q.y);
+    a1 = make_float2(q.z, q.w);
   }
-  __device__ __forceinline__ float2 get(int ii, int c) const {
-    return ld_f2(rc, (vrow + (unsigned)ii) * rowb + (c ? o1 : o0), 0);
+  __device__ __forceinline__ void get2(int ii, float2& a0, float2& a1) const {
+    const float4 q = ld_f4(rc, (vrow + (unsigned)ii) * rowb + oq, 0);
+    a0 = make_float2(q.x, q.y);
+    a1 = make_float2(q.z, q.w);
   }
 };
 
-// Rows from the block's LDS window (row stride kBW); every index is clamped
-// into the window, which only changes lanes outside the view.
+// Rows from the block's LDS window (row stride kBW, entries of bins
+// rlo - 1 ...); every index is clamped into the window, which only changes
+// lanes outside the view.
 template <int ZB>
 struct FetchLds {
   const float2* t;
@@ -378,10 +395,14 @@ struct FetchLds {
     p0 = t + r * kBW + c0;
     p1 = t + r * kBW + c1;
   }
-  __device__ __forceinline__ float2 at(int k, int c) const { return (c ? p1 : p0)[k * kBW]; }
-  __device__ __forceinline__ float2 get(int ii, int c) const {
+  __device__ __forceinline__ void pair(int k, float2& a0, float2& a1) const {
+    a0 = p0[k * kBW];
+    a1 = p1[k * kBW];
+  }
+  __device__ __forceinline__ void get2(int ii, float2& a0, float2& a1) const {
     const int r = min(max(ii - lo, 0), hr);
-    return t[r * kBW + (c ? c1 : c0)];
+    a0 = t[r * kBW + c0];
+    a1 = t[r * kBW + c1];
   }
 };
 
@@ -452,7 +473,7 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
   // active corner columns of the block
   const float cx0 = (float)(bx * 16) - G.RESM, cx1 = (float)min(bx * 16 + 15, G.res - 1) - G.RESM;
   const float cy0 = (float)(by * 16) - G.RESM, cy1 = (float)min(by * 16 + 15, G.res - 1) - G.RESM;
-  const unsigned rowb = (unsigned)G.nrad * 8u;
+  const unsigned rowb = (unsigned)packed_row_len(G.nrad) * 8u;
   const unsigned nrows = (unsigned)(G.nsinogs + 2 * G.nseg);
   const float fz0 = (float)zb, fzl = (float)(zb + ZB - 1);
   float n[ZB], d[ZB];
@@ -475,7 +496,7 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
       const int r = lr + 8 * k;
       pre[k] = make_float2(0.f, 0.f);
       if (r < nrr && lc < nbb)
-        pre[k] = ld_f2(rc2, (vrow + (unsigned)(lo + r)) * rowb + (unsigned)(w.z + lc) * 8u, 0);
+        pre[k] = ld_f2(rc2, (vrow + (unsigned)(lo + r)) * rowb + (unsigned)(w.z + 1 + lc) * 8u, 0);
     }
   };
   auto commit = [&](const int4 w, int buf) {
@@ -580,7 +601,7 @@ __global__ __launch_bounds__(256) void k_back_direct(DevGeom G, const float2* __
   const bool act = ix < G.res && iy < G.res;
   const __amdgpu_buffer_rsrc_t rc2 = rsrc(c2, c2_bytes);
   const float xv = (float)ix - G.RESM, yv = (float)iy - G.RESM;
-  const unsigned rowb = (unsigned)G.nrad * 8u;
+  const unsigned rowb = (unsigned)packed_row_len(G.nrad) * 8u;
   const unsigned nr = (unsigned)(G.nsinogs + 2 * G.nseg);
   const float fz0 = (float)zb, fzl = (float)(zb + ZB - 1);
   float n[ZB], d[ZB];
@@ -599,8 +620,7 @@ __global__ __launch_bounds__(256) void k_back_direct(DevGeom G, const float2* __
       f.rc = rc2;
       f.vrow = (unsigned)a * nr + segrow;
       f.rowb = rowb;
-      f.o0 = (unsigned)v.r0 * 8u;
-      f.o1 = (unsigned)v.r1 * 8u;
+      f.oq = (unsigned)(v.iq + 1) * 8u;
       accumulate_view<ZB>(v, se.fzb, v.yr, zb, G.nzs, fzm, fzhi, f, n, d);
     }
   }
@@ -734,7 +754,7 @@ int back_window_rows(const DevGeom& G, int zb) {
 
 size_t packed_bytes(const DevGeom& G, int subset, int nsub) {
   return sizeof(float2) * (size_t)nav_of(G, subset, nsub) * (G.nsinogs + 2 * (size_t)G.nseg) *
-         G.nrad;
+         packed_row_len(G.nrad);
 }
 
 hipError_t launch_pack(const DevGeom& G, const float* corr, const uint8_t* mask, float2* c2,

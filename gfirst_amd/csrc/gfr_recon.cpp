@@ -75,7 +75,8 @@ Plan plan_of(const GeomHost& h, const gfr_recon_params* p, bool bgD, bool bgT) {
   pl.mask = (size_t)h.info.ndata;
   pl.llh = sizeof(double) * (size_t)llh_capacity(h, p->nsub) + 64;
   const size_t nav = (size_t)(h.g.nang + p->nsub - 1) / p->nsub;
-  pl.corr2 = sizeof(float2) * nav * ((size_t)h.info.nsinogs + 2 * (size_t)h.g.nseg) * h.g.nrad;
+  pl.corr2 = sizeof(float2) * nav * ((size_t)h.info.nsinogs + 2 * (size_t)h.g.nseg) *
+             ((size_t)h.g.nrad + 2);  // packed rows: NRAD + 2 entries
   pl.rot = sizeof(float) * (size_t)(h.info.lambda1 - h.info.lambda0 + 1) * h.g.nzs * h.g.nrad *
            std::min(nav, (size_t)kRotChunk);
   const bool two = p->mode >= 2;
