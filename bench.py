@@ -195,7 +195,8 @@ def run_c1(args):
                      "fp32_frac": round(flops / PEAK_FP32_TFLOPS, 4)},
         "loglik_last_pass": ll,
         "tiles": {"back_slices_per_thread": tune[0], "zsum_sinograms_per_wave": tune[1],
-                  "back_lds_window": tune[5], "autotuned": tune[2]},
+                  "back_lds_window": tune[5], "back_waves_per_simd": tune[6],
+                  "autotuned": tune[2]},
     }
     r.close()
     ctx.close()

@@ -92,8 +92,8 @@ SIGNATURES = {
     "gfr_memcpy_d2h": (_I, [_V, _V, _V, _SZ, _V]),
     "gfr_zero": (_I, [_V, _V, _SZ, _V]),
     "gfr_tuning_get": (_I, [_V, _G, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I),
-                            C.POINTER(_F), C.POINTER(_F)]),
-    "gfr_tuning_set": (_I, [_V, _G, _I, _I, _I, _I]),
+                            C.POINTER(_I), C.POINTER(_F), C.POINTER(_F)]),
+    "gfr_tuning_set": (_I, [_V, _G, _I, _I, _I, _I, _I]),
     "gfr_fill_cylinder": (_I, [_V, _G, _F, _V, _V]),
     "gfr_blur_xy": (_I, [_V, _G, _F, _V, _V, _V]),
     "gfr_forward_project": (_I, [_V, _G, _V, _V, _I, _I, _V]),

@@ -114,17 +114,18 @@ class Context:
 
     # -- tile shapes of the projectors -----------------------------------------
     def tuning(self, g, nsub):
-        """(back_zb, zsum_sg, measured, ms_back, ms_zsum, back_lds) for (geometry, nsub)."""
-        zb, sg, ld = C.c_int(), C.c_int(), C.c_int()
+        """(back_zb, zsum_sg, measured, ms_back, ms_zsum, back_lds, back_waves) for
+        (geometry, nsub)."""
+        zb, sg, ld, bw = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         mb, mz = C.c_float(), C.c_float()
         rc = _check(self.h, lib().gfr_tuning_get(self.h, C.byref(g.c()), nsub, C.byref(zb),
-                                                 C.byref(sg), C.byref(ld), C.byref(mb),
-                                                 C.byref(mz)))
-        return zb.value, sg.value, bool(rc), mb.value, mz.value, bool(ld.value)
+                                                 C.byref(sg), C.byref(ld), C.byref(bw),
+                                                 C.byref(mb), C.byref(mz)))
+        return zb.value, sg.value, bool(rc), mb.value, mz.value, bool(ld.value), bw.value
 
-    def set_tuning(self, g, nsub, back_zb, zsum_sg, back_lds=False):
+    def set_tuning(self, g, nsub, back_zb, zsum_sg, back_lds=False, back_waves=0):
         _check(self.h, lib().gfr_tuning_set(self.h, C.byref(g.c()), nsub, back_zb, zsum_sg,
-                                            1 if back_lds else 0))
+                                            1 if back_lds else 0, back_waves))
 
     # -- operators (one per reference wrapper) ---------------------------------
     def fill_cylinder(self, g, val, out=None):

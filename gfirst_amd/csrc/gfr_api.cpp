@@ -605,7 +605,7 @@ int gfr_vlor_destroy(gfr_vlor* v) {
 extern "C" {
 
 int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int* back_zb, int* zsum_sg,
-                   int* back_lds, float* ms_back, float* ms_zsum) {
+                   int* back_lds, int* back_waves, float* ms_back, float* ms_zsum) {
   GFR_CHECK_CTX(ctx);
   GFR_GEOM(ctx, g, gd);
   if (nsub < 1 || nsub > g->nang) return set_err(ctx, GFR_EINVAL, "nsub out of range");
@@ -613,24 +613,28 @@ int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int* back_zb, 
   if (back_zb) *back_zb = t.back_zb;
   if (zsum_sg) *zsum_sg = t.zsum_sg;
   if (back_lds) *back_lds = t.back_lds ? 1 : 0;
+  if (back_waves) *back_waves = t.back_waves;
   if (ms_back) *ms_back = t.ms_back;
   if (ms_zsum) *ms_zsum = t.ms_zsum;
   return t.tuned ? 1 : 0;
 }
 
 int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int back_zb, int zsum_sg,
-                   int back_lds) {
+                   int back_lds, int back_waves) {
   GFR_CHECK_CTX(ctx);
   GFR_GEOM(ctx, g, gd);
   if (nsub < 1 || nsub > g->nang) return set_err(ctx, GFR_EINVAL, "nsub out of range");
   auto ok = [](int v) { return v == 4 || v == 8 || v == 16; };
   if (!ok(back_zb) || !ok(zsum_sg)) return set_err(ctx, GFR_EINVAL, "tile sizes must be 4, 8 or 16");
   if (back_lds != 0 && back_lds != 1) return set_err(ctx, GFR_EINVAL, "back_lds must be 0 or 1");
+  if (back_waves != 0 && !(back_waves == 6 && back_zb == 8 && !back_lds))
+    return set_err(ctx, GFR_EINVAL, "back_waves: 0, or 6 with 8 slices and direct reads");
   std::lock_guard<std::mutex> lk(ctx->mu);
   Tune& t = gd->tune[nsub];
   t.back_zb = back_zb;
   t.zsum_sg = zsum_sg;
   t.back_lds = back_lds != 0;
+  t.back_waves = back_waves;
   t.tuned = true;
   t.ms_back = t.ms_zsum = 0.f;
   return GFR_OK;

@@ -49,6 +49,7 @@ struct Tune {
   int back_zb = 8;   // slices per backprojector thread: 4, 8 or 16
   int zsum_sg = 8;   // consecutive sinograms per z-sum wave: 4, 8 or 16
   bool back_lds = false;  // backprojector reads its rows through an LDS window
+  int back_waves = 0;     // register budget for this many waves per SIMD (0: compiler's)
   bool tuned = false;
   float ms_back = 0.f, ms_zsum = 0.f;
 };

@@ -587,8 +587,11 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
 
 // Same sums with every read from global memory: geometries whose window does
 // not fit the LDS bounds.
-template <int ZB>
-__global__ __launch_bounds__(256) void k_back_direct(DevGeom G, const float2* __restrict__ c2,
+// MINW > 0 asks the compiler for a register budget that keeps MINW waves per
+// SIMD resident (an autotuner candidate; more resident waves can also thrash
+// the L1).
+template <int ZB, int MINW>
+__global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_back_direct(DevGeom G, const float2* __restrict__ c2,
                                                      unsigned c2_bytes, float* __restrict__ numer,
                                                      float* __restrict__ denom, int subset,
                                                      int nsub, const int* __restrict__ dsub) {
@@ -771,7 +774,7 @@ hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* nume
   if (bytes >= 0xFFFFFFF0u) return hipErrorInvalidValue;
   const unsigned nbx = (G.res + 15) / 16;
   const bool bins_fit = 21.3f * G.fxb + 3.f <= (float)kBinsMax;
-#define GFR_BK(ZB)                                                                              \
+#define GFR_BK(ZB, W)                                                                           \
   {                                                                                             \
     const int rmax = back_window_rows(G, ZB);                                                   \
     if (tune.back_lds && bins_fit && rmax > 0) {                                                \
@@ -779,14 +782,21 @@ hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* nume
       hipLaunchKernelGGL(k_back<ZB>, dim3(nb), dim3(256), sizeof(float2) * 2 * rmax * kBW, st,  \
                          G, corr2, (unsigned)bytes, numer, denom, subset, nsub, dsub, rmax);    \
     } else {                                                                                    \
-      hipLaunchKernelGGL(k_back_direct<ZB>, dim3(nbx, nbx, (G.nzs + ZB - 1) / ZB), dim3(256), 0, \
-                         st, G, corr2, (unsigned)bytes, numer, denom, subset, nsub, dsub);      \
+      hipLaunchKernelGGL((k_back_direct<ZB, W>), dim3(nbx, nbx, (G.nzs + ZB - 1) / ZB),         \
+                         dim3(256), 0, st, G, corr2, (unsigned)bytes, numer, denom, subset,     \
+                         nsub, dsub);                                                           \
     }                                                                                           \
   }
   switch (tune.back_zb) {
-    case 4: GFR_BK(4); break;
-    case 16: GFR_BK(16); break;
-    default: GFR_BK(8); break;
+    case 4: GFR_BK(4, 0); break;
+    case 16: GFR_BK(16, 0); break;
+    default:
+      if (tune.back_waves == 6) {
+        GFR_BK(8, 6);
+      } else {
+        GFR_BK(8, 0);
+      }
+      break;
   }
 #undef GFR_BK
   return hipGetLastError();
@@ -835,19 +845,22 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
       t.zsum_sg = c;
     }
   }
-  for (int lds = 0; lds < 2; lds++) {
-    for (int c : cand) {
-      Tune tt = t;
-      tt.back_zb = c;
-      tt.back_lds = lds != 0;
-      const float ms = time_it([&] {
-        return launch_back_packed(G, corr2, num, den, 0, nsub, nullptr, tt, st);
-      });
-      if (ms < bb) {
-        bb = ms;
-        t.back_zb = c;
-        t.back_lds = lds != 0;
-      }
+  // backprojector: (kind, slices, register budget)
+  const int bcand[7][3] = {{0, 4, 0}, {0, 8, 0}, {0, 8, 6}, {0, 16, 0},
+                           {1, 4, 0}, {1, 8, 0}, {1, 16, 0}};
+  for (const auto& c : bcand) {
+    Tune tt = t;
+    tt.back_lds = c[0] != 0;
+    tt.back_zb = c[1];
+    tt.back_waves = c[2];
+    const float ms = time_it([&] {
+      return launch_back_packed(G, corr2, num, den, 0, nsub, nullptr, tt, st);
+    });
+    if (ms < bb) {
+      bb = ms;
+      t.back_lds = tt.back_lds;
+      t.back_zb = tt.back_zb;
+      t.back_waves = tt.back_waves;
     }
   }
   (void)hipEventDestroy(e0);

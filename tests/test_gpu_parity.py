@@ -135,14 +135,15 @@ def test_back_project_c1_subset_vs_oracle(gctx, orc):
     close(d2, de, rtol=0, what="denom c1 lds")
 
 
-@pytest.mark.parametrize("zb,sg,lds", [(4, 4, False), (8, 16, False), (16, 8, False),
-                                       (4, 8, True), (8, 4, True), (16, 16, True)])
-def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg, lds):
+@pytest.mark.parametrize("zb,sg,lds,bw", [(4, 4, False, 0), (8, 16, False, 0),
+                                          (16, 8, False, 0), (8, 8, False, 6),
+                                          (4, 8, True, 0), (8, 4, True, 0), (16, 16, True, 0)])
+def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg, lds, bw):
     """Every autotuner candidate gives the oracle's bits (forward and back)."""
     g = C0
-    gctx.set_tuning(g, 4, zb, sg, lds)
+    gctx.set_tuning(g, 4, zb, sg, lds, bw)
     t = gctx.tuning(g, 4)
-    assert t[:3] == (zb, sg, True) and t[5] == lds
+    assert t[:3] == (zb, sg, True) and t[5] == lds and t[6] == bw
     x = rand_img(g, 30)
     close(gctx.forward_project(g, x, 2, 4), orc.forward(g, x, 2, 4), what="fwd sg=%d" % sg)
     rng = np.random.default_rng(31)
