@@ -139,7 +139,7 @@ SIGNATURES = {
     "gfr_batch_stats": (_I, [_V, C.POINTER(_LL), C.POINTER(_LL)]),
     "gfr_batch_stream": (_V, [_V]),
     "gfr_batch_destroy": (_I, [_V]),
-    "gfr_server_create": (_I, [_V, _I, _I, C.POINTER(_V)]),
+    "gfr_server_create": (_I, [_V, _I, _I, _I, C.POINTER(_V)]),
     "gfr_server_submit": (_I, [_V, _G, C.POINTER(GfrReconParams), _V, _V, _V, _V, _LL,
                                C.POINTER(_LL)]),
     "gfr_server_info": (_I, [_V, _LL, C.POINTER(GfrRequestInfo)]),

@@ -233,8 +233,8 @@ class Context:
     def batch(self, g, params, norm):
         return Batch(self, g, params, norm)
 
-    def server(self, max_active=32, inflight=2):
-        return Server(self, max_active, inflight)
+    def server(self, max_active=32, inflight=2, admitters=4):
+        return Server(self, max_active, inflight, admitters)
 
     def poisson(self, mean, scale, seed, out=None, mean2=None, scale2=0.0):
         """Seeded Poisson(scale * mean + scale2 * mean2) draws on the device
@@ -486,10 +486,11 @@ class Server:
     numpy arrays or DeviceArrays; they are kept referenced here until the
     request has been admitted."""
 
-    def __init__(self, ctx, max_active=32, inflight=2):
+    def __init__(self, ctx, max_active=32, inflight=2, admitters=4):
         self.ctx = ctx
         h = C.c_void_p()
-        _check(ctx.h, lib().gfr_server_create(ctx.h, max_active, inflight, C.byref(h)))
+        _check(ctx.h, lib().gfr_server_create(ctx.h, max_active, inflight, admitters,
+                                              C.byref(h)))
         self.h = h.value
         self._inputs = {}
         self._meta = {}

@@ -474,100 +474,7 @@ int gfr_image_max(gfr_ctx* ctx, const gfr_geometry* g, const float* d_x, float* 
 int gfr_vlor_create(gfr_ctx* ctx, const gfr_geometry* g, const int32_t* rec, long long nrec,
                     const float* d_yT, gfr_vlor** out) {
   GFR_CHECK_CTX(ctx);
-  GFR_GEOM(ctx, g, gd);
-  GFR_NONNULL(ctx, out, "out");
-  GFR_NONNULL(ctx, d_yT, "d_yT");
-  if (nrec < 0 || (nrec > 0 && !rec)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: records");
-  if (nrec > (1LL << 30)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: too many records");
-  *out = nullptr;
-  const long long nd = gd->host.info.ndata;
-  // Valid records (GFIRST.cu:235-237): 1-based indices, 0-based i must satisfy
-  // 0 < i < NDATA for both LORs.
-  std::vector<int> i1, i2;
-  i1.reserve(nrec);
-  i2.reserve(nrec);
-  std::vector<float> v0(nd, 0.f);
-  for (long long e = 0; e < nrec; e++) {
-    const long long a = (long long)rec[2 * e] - 1, b = (long long)rec[2 * e + 1] - 1;
-    if (a <= 0 || b <= 0 || a >= nd || b >= nd) continue;
-    i1.push_back((int)a);
-    i2.push_back((int)b);
-    v0[a] += 1.f;  // datos_v0[i1]++ (GFIRST.cu:238-239)
-    v0[b] += 1.f;
-  }
-  const long long nev = (long long)i1.size();
-  // Bin -> record lists in record order (stable counting sort over touched bins).
-  std::vector<std::pair<long long, int>> ent;
-  ent.reserve(2 * nev);
-  for (long long e = 0; e < nev; e++) {
-    ent.emplace_back(i1[e], (int)e);
-    ent.emplace_back(i2[e], (int)e);
-  }
-  std::stable_sort(ent.begin(), ent.end(),
-                   [](const std::pair<long long, int>& x, const std::pair<long long, int>& y) {
-                     return x.first < y.first;
-                   });
-  std::vector<long long> bins;
-  std::vector<int> start, evs;
-  evs.reserve(ent.size());
-  for (size_t k = 0; k < ent.size(); k++) {
-    if (k == 0 || ent[k].first != ent[k - 1].first) {
-      bins.push_back(ent[k].first);
-      start.push_back((int)k);
-    }
-    evs.push_back(ent[k].second);
-  }
-  start.push_back((int)ent.size());
-  gfr_vlor* v = new gfr_vlor();
-  v->ctx = ctx;
-  v->g = *g;
-  v->nev = nev;
-  v->nbins = (long long)bins.size();
-  hipStream_t st = ctx->stream;
-  auto dalloc = [&](void** p, size_t b) -> hipError_t {
-    v->bytes += b;
-    return hipMalloc(p, b ? b : 4);
-  };
-  float* d_v0 = nullptr;
-  hipError_t e = hipSuccess;
-  if (e == hipSuccess) e = dalloc((void**)&v->d_i1, sizeof(int) * nev);
-  if (e == hipSuccess) e = dalloc((void**)&v->d_i2, sizeof(int) * nev);
-  if (e == hipSuccess) e = dalloc((void**)&v->d_blankt, sizeof(float) * nd);
-  if (e == hipSuccess) e = dalloc((void**)&v->d_sinv, sizeof(float) * nev);
-  if (e == hipSuccess) e = dalloc((void**)&v->d_valor, sizeof(float) * nev);
-  if (e == hipSuccess) e = dalloc((void**)&v->d_bins, sizeof(long long) * bins.size());
-  if (e == hipSuccess) e = dalloc((void**)&v->d_start, sizeof(int) * start.size());
-  if (e == hipSuccess) e = dalloc((void**)&v->d_ev, sizeof(int) * evs.size());
-  if (e == hipSuccess) e = hipMalloc((void**)&d_v0, sizeof(float) * nd);
-  if (e == hipSuccess && nev) {
-    e = hipMemcpy(v->d_i1, i1.data(), sizeof(int) * nev, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(v->d_i2, i2.data(), sizeof(int) * nev, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hipMemcpy(v->d_bins, bins.data(), sizeof(long long) * bins.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(v->d_ev, evs.data(), sizeof(int) * evs.size(), hipMemcpyHostToDevice);
-  }
-  if (e == hipSuccess)
-    e = hipMemcpy(v->d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d_v0, v0.data(), sizeof(float) * nd, hipMemcpyHostToDevice);
-  // BLANKT = T/(RES*v0) where v0 > 0.01 (GFIRST.cu:244-247); T already x RES.
-  if (e == hipSuccess) e = launch_vlor_blankt(d_yT, d_v0, (float)g->res, v->d_blankt, nd, st);
-  // valor per record is static: BLANKT[i1] + BLANKT[i2].
-  if (e == hipSuccess)
-    e = launch_vlor_event(v->d_i1, v->d_i2, v->d_blankt, v->d_blankt, v->d_sinv, v->d_valor, nev,
-                          1, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  if (d_v0) (void)hipFree(d_v0);
-  if (e != hipSuccess) {
-    v->ctx = nullptr;  // its bytes were never counted in the context
-    gfr_vlor_destroy(v);
-    return hip_err(ctx, e, "gfr_vlor_create");
-  }
-  {
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    ctx->in_use += v->bytes;
-  }
-  *out = v;
-  return GFR_OK;
+  return vlor_create_on(ctx, g, rec, nrec, d_yT, ctx->stream, out);
 }
 
 int gfr_vlor_assign(gfr_ctx* ctx, gfr_vlor* v, const float* d_proj, float* d_dv, int subset,
@@ -641,3 +548,118 @@ int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int back_zb, i
 }
 
 }  // extern "C"
+
+namespace gfr {
+
+int vlor_create_on(gfr_ctx* ctx, const gfr_geometry* g, const int32_t* rec, long long nrec,
+                   const float* d_yT, hipStream_t st, gfr_vlor** out) {
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, out, "out");
+  GFR_NONNULL(ctx, d_yT, "d_yT");
+  if (nrec < 0 || (nrec > 0 && !rec)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: records");
+  if (nrec > (1LL << 30)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: too many records");
+  *out = nullptr;
+  const long long nd = gd->host.info.ndata;
+  if (nd >= (1LL << 31)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: sinogram too large");
+  // Valid records (GFIRST.cu:235-237): 1-based indices, 0-based i must satisfy
+  // 0 < i < NDATA for both LORs.
+  std::vector<int> i1, i2;
+  i1.reserve(nrec);
+  i2.reserve(nrec);
+  for (long long e = 0; e < nrec; e++) {
+    const long long a = (long long)rec[2 * e] - 1, b = (long long)rec[2 * e + 1] - 1;
+    if (a <= 0 || b <= 0 || a >= nd || b >= nd) continue;
+    i1.push_back((int)a);
+    i2.push_back((int)b);
+  }
+  const long long nev = (long long)i1.size();
+  // Bin -> record lists in record order: entries (bin, event) in the order
+  // (i1[0], 0), (i2[0], 0), (i1[1], 1), ... sorted by bin with a stable LSD
+  // radix sort (11-bit digits).
+  const size_t ne = 2 * (size_t)nev;
+  std::vector<int> key(ne), val(ne), k2(ne), v2(ne);
+  for (long long e = 0; e < nev; e++) {
+    key[2 * e] = i1[e];
+    key[2 * e + 1] = i2[e];
+    val[2 * e] = val[2 * e + 1] = (int)e;
+  }
+  for (int shift = 0; shift < 31 && (nd - 1) >> shift; shift += 11) {
+    std::vector<size_t> cnt(2049, 0);
+    for (size_t k = 0; k < ne; k++) cnt[((unsigned)key[k] >> shift & 2047u) + 1]++;
+    for (int d = 0; d < 2048; d++) cnt[d + 1] += cnt[d];
+    for (size_t k = 0; k < ne; k++) {
+      const size_t o = cnt[(unsigned)key[k] >> shift & 2047u]++;
+      k2[o] = key[k];
+      v2[o] = val[k];
+    }
+    key.swap(k2);
+    val.swap(v2);
+  }
+  std::vector<long long> bins;
+  std::vector<int> start;
+  for (size_t k = 0; k < ne; k++) {
+    if (k == 0 || key[k] != key[k - 1]) {
+      bins.push_back(key[k]);
+      start.push_back((int)k);
+    }
+  }
+  start.push_back((int)ne);
+  gfr_vlor* v = new gfr_vlor();
+  v->ctx = ctx;
+  v->g = *g;
+  v->nev = nev;
+  v->nbins = (long long)bins.size();
+  auto dalloc = [&](void** p, size_t b) -> hipError_t {
+    v->bytes += b;
+    return hipMalloc(p, b ? b : 4);
+  };
+  float* d_v0 = nullptr;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = dalloc((void**)&v->d_i1, sizeof(int) * nev);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_i2, sizeof(int) * nev);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_blankt, sizeof(float) * nd);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_sinv, sizeof(float) * nev);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_valor, sizeof(float) * nev);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_bins, sizeof(long long) * bins.size());
+  if (e == hipSuccess) e = dalloc((void**)&v->d_start, sizeof(int) * start.size());
+  if (e == hipSuccess) e = dalloc((void**)&v->d_ev, sizeof(int) * ne);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_v0, sizeof(float) * nd);
+  // host sources stay alive until the synchronisation below
+  if (e == hipSuccess && nev) {
+    e = hipMemcpyAsync(v->d_i1, i1.data(), sizeof(int) * nev, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(v->d_i2, i2.data(), sizeof(int) * nev, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(v->d_bins, bins.data(), sizeof(long long) * bins.size(),
+                         hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(v->d_ev, val.data(), sizeof(int) * ne, hipMemcpyHostToDevice, st);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(v->d_start, start.data(), sizeof(int) * start.size(),
+                       hipMemcpyHostToDevice, st);
+  // datos_v0 counts on the device (GFIRST.cu:238-239)
+  if (e == hipSuccess) e = launch_zero(d_v0, sizeof(float) * nd, st);
+  if (e == hipSuccess) e = launch_vlor_count(v->d_i1, v->d_i2, nev, d_v0, st);
+  // BLANKT = T/(RES*v0) where v0 > 0.01 (GFIRST.cu:244-247); T already x RES.
+  if (e == hipSuccess) e = launch_vlor_blankt(d_yT, d_v0, (float)g->res, v->d_blankt, nd, st);
+  // valor per record is static: BLANKT[i1] + BLANKT[i2].
+  if (e == hipSuccess)
+    e = launch_vlor_event(v->d_i1, v->d_i2, v->d_blankt, v->d_blankt, v->d_sinv, v->d_valor, nev,
+                          1, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (d_v0) (void)hipFree(d_v0);
+  if (e != hipSuccess) {
+    v->ctx = nullptr;  // its bytes were never counted in the context
+    gfr_vlor_destroy(v);
+    return hip_err(ctx, e, "gfr_vlor_create");
+  }
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->in_use += v->bytes;
+  }
+  *out = v;
+  return GFR_OK;
+}
+
+}  // namespace gfr

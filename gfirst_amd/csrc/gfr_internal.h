@@ -208,6 +208,8 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
                        hipStream_t st, Tune* out);
 
 // VLOR.
+hipError_t launch_vlor_count(const int* i1, const int* i2, long long nev, float* v0,
+                             hipStream_t st);
 hipError_t launch_vlor_blankt(const float* yT, const float* v0, float res, float* blankt,
                               long long nd, hipStream_t st);
 hipError_t launch_vlor_event(const int* i1, const int* i2, const float* blankt,
@@ -220,6 +222,12 @@ hipError_t launch_vlor_bins(const DevGeom& G, const long long* bins, const int* 
                             const float* sinv, const float* valor, float* dv, int subset,
                             int nsub, hipStream_t st);
 
+}  // namespace gfr
+
+namespace gfr {
+// gfr_vlor_create with its device work ordered on stream st.
+int vlor_create_on(gfr_ctx* ctx, const gfr_geometry* g, const int32_t* rec, long long nrec,
+                   const float* d_yT, hipStream_t st, gfr_vlor** out);
 }  // namespace gfr
 
 struct gfr_vlor {

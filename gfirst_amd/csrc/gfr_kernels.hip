@@ -283,6 +283,17 @@ __global__ void k_vlor_blankt(const float* __restrict__ yT, const float* __restr
   blankt[i] = ((double)val > 0.01) ? yT[i] / (res * val) : 0.f;
 }
 
+// datos_v0[i]++ for both LORs of every valid record (GFIRST.cu:238-239):
+// counts stay exact integers in float (below 2^24), so the order of the
+// atomic additions does not matter.
+__global__ void k_vlor_count(const int* __restrict__ i1, const int* __restrict__ i2,
+                             long long nev, float* __restrict__ v0) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nev) return;
+  atomicAdd(&v0[i1[e]], 1.f);
+  atomicAdd(&v0[i2[e]], 1.f);
+}
+
 // Per event: s_inv = 1/(s1+s2) in double, rounded to float (GFIRST.cu:384-385)
 // and, once, valor = BLANKT[i1] + BLANKT[i2] (GFIRST.cu:381).
 __global__ void k_vlor_event(const int* __restrict__ i1, const int* __restrict__ i2,
@@ -495,6 +506,14 @@ hipError_t launch_vlor_blankt(const float* yT, const float* v0, float res, float
                               long long nd, hipStream_t st) {
   hipLaunchKernelGGL(k_vlor_blankt, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st, yT, v0,
                      res, blankt, nd);
+  return hipGetLastError();
+}
+
+hipError_t launch_vlor_count(const int* i1, const int* i2, long long nev, float* v0,
+                             hipStream_t st) {
+  if (nev == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vlor_count, dim3((unsigned)((nev + 255) / 256)), dim3(256), 0, st, i1, i2,
+                     nev, v0);
   return hipGetLastError();
 }
 

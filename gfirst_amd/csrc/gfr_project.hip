@@ -41,22 +41,27 @@ __device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, unsigned voff,
 }
 
 // ---------------------------------------------------------------------------
-// k_rotate: one thread per (angle a of the chunk, sample l, plane z, bin r).
-// Logical block order (r-block, l-block, z, a) with XCD-contiguous ranges, so
-// the work-groups sharing an image plane share an L2.
+// k_rotate: one thread per (angle a of the chunk, sample l, bin r) and a run
+// of kRotZ planes: the sample's x/y position and weights do not depend on
+// the plane, so they are computed once and the planes' four texels are
+// gathered at fixed offsets. Logical block order (r-block, l-block,
+// plane-run, a) with XCD-contiguous ranges.
 // ---------------------------------------------------------------------------
+constexpr int kRotZ = 16;
+
 __global__ __launch_bounds__(256) void k_rotate(DevGeom G, const float* __restrict__ img,
                                                 float* __restrict__ R, int subset, int nsub,
                                                 const int* __restrict__ dsub, int a0, int nl) {
   if (dsub) subset = *dsub;
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const unsigned nrb = (G.nrad + kWave - 1) / kWave, nlb = (nl + 3) / 4;
+  const unsigned nzb = (G.nzs + kRotZ - 1) / kRotZ;
   const unsigned rb = b % nrb;
   unsigned q = b / nrb;
   const unsigned lb = q % nlb;
   q /= nlb;
-  const int z = (int)(q % G.nzs);
-  const int a = (int)(q / G.nzs);
+  const int zb = (int)(q % nzb) * kRotZ;
+  const int a = (int)(q / nzb);
   const int r = rb * kWave + threadIdx.x;
   const int li = lb * 4 + threadIdx.y;
   if (r >= G.nrad || li >= nl || a0 + a >= (G.nang - subset + nsub - 1) / nsub) return;
@@ -72,10 +77,20 @@ __global__ __launch_bounds__(256) void k_rotate(DevGeom G, const float* __restri
   float ax, ay;
   axis(x, G.res, x0i, x1i, ax);
   axis(y, G.res, y0i, y1i, ay);
-  const float* p = img + (size_t)z * G.res * G.res;
-  const float v = lerp1(lerp1(p[y0i * G.res + x0i], p[y0i * G.res + x1i], ax),
-                        lerp1(p[y1i * G.res + x0i], p[y1i * G.res + x1i], ax), ay);
-  R[(((size_t)a * nl + li) * G.nzs + z) * G.nrad + r] = v;
+  const int o00 = y0i * G.res + x0i, o01 = y0i * G.res + x1i;
+  const int o10 = y1i * G.res + x0i, o11 = y1i * G.res + x1i;
+  const size_t plane = (size_t)G.res * G.res;
+  const float* p = img + (size_t)zb * plane;
+  float* out = R + (((size_t)a * nl + li) * G.nzs + zb) * G.nrad + r;
+  const int nz = min(kRotZ, G.nzs - zb);
+#pragma unroll
+  for (int k = 0; k < kRotZ; k++) {
+    if (k < nz) {
+      const float* pk = p + (size_t)k * plane;
+      out[(size_t)k * G.nrad] =
+          lerp1(lerp1(pk[o00], pk[o01], ax), lerp1(pk[o10], pk[o11], ax), ay);
+    }
+  }
 }
 
 // Where the forward kernel writes.
@@ -723,7 +738,7 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
   for (int a0 = 0; a0 < nav; a0 += chunk) {
     const int na = min(chunk, nav - a0);
     const unsigned rbytes = (unsigned)((size_t)na * per * sizeof(float));
-    const unsigned grot = nrb * ((nl + 3) / 4) * G.nzs * na;
+    const unsigned grot = nrb * ((nl + 3) / 4) * ((G.nzs + kRotZ - 1) / kRotZ) * na;
     hipLaunchKernelGGL(k_rotate, dim3(grot), dim3(kWave, 4), 0, st, G, img, R, subset, nsub, dsub,
                        a0, nl);
     hipError_t e = hipGetLastError();

@@ -301,7 +301,7 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
     if (e == hipSuccess) e = launch_scale_copy(r->ptc, (float)g->res, r->dv, nd, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create vlor"));
-    int rc = gfr_vlor_create(ctx, g, rec, nrec, r->dv, &r->vl);
+    int rc = vlor_create_on(ctx, g, rec, nrec, r->dv, st, &r->vl);
     if (rc != GFR_OK) return rfail(r, rc);
     // Projection cache of the triples estimate (bins outside the current
     // subset keep their last projection).

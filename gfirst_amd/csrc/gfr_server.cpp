@@ -52,6 +52,8 @@ struct Req {
 struct gfr_server {
   gfr_ctx* ctx = nullptr;
   int max_active = 0, inflight = 2;
+  int creating = 0;       // sessions being created by admitters
+  bool mem_wait = false;  // a request did not fit: admit nothing until a retirement
   std::chrono::steady_clock::time_point t0;
   std::mutex mu;
   std::condition_variable cv_admit, cv_step, cv_retire, cv_done;
@@ -63,7 +65,8 @@ struct gfr_server {
   bool stop_retire = false;   // retirer (after the last sessions are queued)
   long long n_done = 0, n_failed = 0, n_passes = 0;
   std::vector<hipEvent_t> ev_pool;  // stepper-owned
-  std::thread th_step, th_admit, th_retire;
+  std::thread th_step, th_retire;
+  std::vector<std::thread> th_admit;
 
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -92,31 +95,41 @@ void fail_req(gfr_server* s, const std::shared_ptr<Req>& q, int code, const std:
 }
 
 // Creates sessions for pending requests in arrival order as slots and
-// budget allow; a request that cannot fit even on an idle server fails with
-// GFR_ENOMEM instead of blocking the queue.
+// budget allow (several admitters run at once, so one large upload does not
+// hold back the requests behind it); a request that cannot fit even on an
+// idle server fails with GFR_ENOMEM instead of blocking the queue.
 void admitter(gfr_server* s) {
   std::unique_lock<std::mutex> lk(s->mu);
   while (!s->stop) {
-    if (s->pending.empty() || (int)s->active.size() >= s->max_active) {
-      s->cv_admit.wait(lk);
+    if (s->pending.empty() || s->mem_wait ||
+        (int)s->active.size() + s->creating >= s->max_active) {
+      s->cv_admit.wait_for(lk, std::chrono::milliseconds(50));
+      if (s->mem_wait && s->active.empty() && s->retire.empty() && s->creating == 0)
+        s->mem_wait = false;  // nothing left to retire: retry (and fail if still too large)
       continue;
     }
     std::shared_ptr<Req> q = s->pending.front();
-    const bool idle = s->active.empty() && s->retire.empty();
+    s->pending.pop_front();
+    s->creating++;
     lk.unlock();
     gfr_recon* r = nullptr;
     int rc = gfr_recon_create(s->ctx, &q->g, &q->p, q->d2, q->t2, q->norm, q->rec, q->nrec, &r);
     std::string msg = rc == GFR_OK ? std::string() : std::string(gfr_last_error(s->ctx));
     lk.lock();
-    if (rc == GFR_ENOMEM && !idle) {
-      // wait for a retirement to free memory, then retry the same request
-      s->cv_admit.wait_for(lk, std::chrono::milliseconds(20));
+    s->creating--;
+    if (rc == GFR_ENOMEM && !(s->active.empty() && s->retire.empty() && s->creating == 0)) {
+      // memory frees up when running sessions retire: back into the queue in
+      // arrival order, and no later request is admitted before it
+      auto it = s->pending.begin();
+      while (it != s->pending.end() && (*it)->id < q->id) ++it;
+      s->pending.insert(it, q);
+      s->mem_wait = true;
       continue;
     }
-    s->pending.pop_front();
     if (rc != GFR_OK) {
       fail_req(s, q, rc, msg);
       s->cv_done.notify_all();
+      s->cv_admit.notify_all();
       continue;
     }
     q->r = r;
@@ -222,8 +235,9 @@ void retirer(gfr_server* s) {
       q->state = GFR_REQ_DONE;
       s->n_done++;
     }
+    s->mem_wait = false;
     s->cv_done.notify_all();
-    s->cv_admit.notify_one();
+    s->cv_admit.notify_all();
   }
 }
 
@@ -231,18 +245,21 @@ void retirer(gfr_server* s) {
 
 extern "C" {
 
-int gfr_server_create(gfr_ctx* ctx, int max_active, int inflight, gfr_server** out) {
+int gfr_server_create(gfr_ctx* ctx, int max_active, int inflight, int admitters,
+                      gfr_server** out) {
   if (!ctx || !out) return GFR_EINVAL;
   *out = nullptr;
-  if (max_active < 1 || max_active > 4096 || inflight < 1 || inflight > 64)
-    return gfr::set_err(ctx, GFR_EINVAL, "max_active in [1, 4096] and inflight in [1, 64] required");
+  if (max_active < 1 || max_active > 4096 || inflight < 1 || inflight > 64 || admitters < 1 ||
+      admitters > 16)
+    return gfr::set_err(ctx, GFR_EINVAL,
+                        "max_active in [1, 4096], inflight in [1, 64], admitters in [1, 16]");
   gfr_server* s = new gfr_server();
   s->ctx = ctx;
   s->max_active = max_active;
   s->inflight = inflight;
   s->t0 = std::chrono::steady_clock::now();
   s->th_step = std::thread(stepper, s);
-  s->th_admit = std::thread(admitter, s);
+  for (int k = 0; k < admitters; k++) s->th_admit.emplace_back(admitter, s);
   s->th_retire = std::thread(retirer, s);
   *out = s;
   return GFR_OK;
@@ -357,7 +374,7 @@ int gfr_server_destroy(gfr_server* s) {
   }
   s->cv_admit.notify_all();
   s->cv_step.notify_all();
-  s->th_admit.join();
+  for (std::thread& t : s->th_admit) t.join();
   s->th_step.join();
   {
     // sessions still admitted: retire them (their results are dropped)

@@ -366,7 +366,9 @@ typedef struct gfr_server_totals {
   long long submitted, pending, active, done, failed, passes;
   double now; /* server clock, seconds */
 } gfr_server_totals;
-int gfr_server_create(gfr_ctx* ctx, int max_active, int inflight, gfr_server** out);
+/* admitters: host threads creating sessions concurrently (uploads, masks). */
+int gfr_server_create(gfr_ctx* ctx, int max_active, int inflight, int admitters,
+                      gfr_server** out);
 /* Queues a request; arguments as gfr_recon_create (d2/t2/norm in host or
  * device memory, device data complete before the call). The pointed-to
  * inputs must stay valid until the request has left GFR_REQ_PENDING. */
