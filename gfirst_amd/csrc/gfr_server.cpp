@@ -53,6 +53,7 @@ struct gfr_server {
   gfr_ctx* ctx = nullptr;
   int max_active = 0, inflight = 2;
   int creating = 0;       // sessions being created by admitters
+  int retiring = 0;       // sessions being read back and freed
   bool mem_wait = false;  // a request did not fit: admit nothing until a retirement
   std::chrono::steady_clock::time_point t0;
   std::mutex mu;
@@ -104,7 +105,8 @@ void admitter(gfr_server* s) {
     if (s->pending.empty() || s->mem_wait ||
         (int)s->active.size() + s->creating >= s->max_active) {
       s->cv_admit.wait_for(lk, std::chrono::milliseconds(50));
-      if (s->mem_wait && s->active.empty() && s->retire.empty() && s->creating == 0)
+      if (s->mem_wait && s->active.empty() && s->retire.empty() && s->creating == 0 &&
+          s->retiring == 0)
         s->mem_wait = false;  // nothing left to retire: retry (and fail if still too large)
       continue;
     }
@@ -117,7 +119,8 @@ void admitter(gfr_server* s) {
     std::string msg = rc == GFR_OK ? std::string() : std::string(gfr_last_error(s->ctx));
     lk.lock();
     s->creating--;
-    if (rc == GFR_ENOMEM && !(s->active.empty() && s->retire.empty() && s->creating == 0)) {
+    if (rc == GFR_ENOMEM &&
+        !(s->active.empty() && s->retire.empty() && s->creating == 0 && s->retiring == 0)) {
       // memory frees up when running sessions retire: back into the queue in
       // arrival order, and no later request is admitted before it
       auto it = s->pending.begin();
@@ -205,15 +208,13 @@ void retirer(gfr_server* s) {
     }
     std::shared_ptr<Req> q = s->retire.front();
     s->retire.pop_front();
+    s->retiring++;
     lk.unlock();
     int rc = q->error;
     std::string msg = q->msg;
     if (rc == GFR_OK) {
-      const gfr_geometry_info inf = [&] {
-        gfr_geometry_info i{};
-        gfr_geometry_check(&q->g, &i);
-        return i;
-      }();
+      gfr_geometry_info inf{};
+      gfr_geometry_check(&q->g, &inf);
       q->xD.resize((size_t)inf.nvox);
       rc = gfr_recon_get(q->r, GFR_OUT_XD, q->xD.data());
       if (rc == GFR_OK && q->p.mode >= 2) {
@@ -222,10 +223,8 @@ void retirer(gfr_server* s) {
       }
       if (rc != GFR_OK) msg = gfr_last_error(s->ctx);
     }
-    gfr_recon_destroy(q->r);  // waits for the session's stream
-    q->r = nullptr;
-    for (hipEvent_t e : q->evs) (void)hipEventDestroy(e);
-    q->evs.clear();
+    // the result is on the host: the request is complete before its session
+    // is freed (so later admissions follow t_done)
     lk.lock();
     s->n_passes += q->passes;
     if (rc != GFR_OK) {
@@ -235,8 +234,15 @@ void retirer(gfr_server* s) {
       q->state = GFR_REQ_DONE;
       s->n_done++;
     }
-    s->mem_wait = false;
     s->cv_done.notify_all();
+    lk.unlock();
+    gfr_recon_destroy(q->r);  // waits for the session's stream
+    q->r = nullptr;
+    for (hipEvent_t e : q->evs) (void)hipEventDestroy(e);
+    q->evs.clear();
+    lk.lock();
+    s->retiring--;
+    s->mem_wait = false;
     s->cv_admit.notify_all();
   }
 }
