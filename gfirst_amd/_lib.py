@@ -102,6 +102,8 @@ SIGNATURES = {
     "gfr_forward_ratio": (_I, [_V, _G, _V, _V, _V, _V, _V, _I, _I, _V]),
     "gfr_back_project": (_I, [_V, _G, _V, _V, _V, _V, _I, _I, _V]),
     "gfr_update": (_I, [_V, _G, _F, _V, _V, _V, _V, _V]),
+    "gfr_sensitivity": (_I, [_V, _G, _V, _V, _I, _I, _V]),
+    "gfr_osem_run": (_I, [_V, _G, C.POINTER(GfrReconParams), _V, _V, _V, _V, _LL, _V, _V]),
     "gfr_jbf3d": (_I, [_V, _G, _V, _V, _F, _I, _F, _F, _V, _V]),
     "gfr_blend": (_I, [_V, _G, _F, _V, _V, _V]),
     "gfr_image_max": (_I, [_V, _G, _V, C.POINTER(_F), _V]),

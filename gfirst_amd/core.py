@@ -202,6 +202,35 @@ class Context:
                                         w.ptr, None))
         return xd.download() if copied else xd
 
+    def sensitivity(self, g, mask, subset=0, nsub=1, out=None):
+        """Calibration half of the backprojection (FP64 accumulation), NVOX doubles."""
+        m, _ = self._dev(mask, dtype=np.uint8)
+        o = out if out is not None else self.array(g.nvox, np.float64)
+        _check(self.h, lib().gfr_sensitivity(self.h, C.byref(g.c()), m.ptr, o.ptr, subset, nsub,
+                                             None))
+        if out is not None:
+            return o
+        self.sync()
+        return o.download()
+
+    def osem_run(self, g, params, d2, t2, norm, records=None):
+        """Whole reconstruction in one call; returns (xD, xT or None)."""
+        d2 = np.ascontiguousarray(d2, np.float32).reshape(-1)
+        norm = np.ascontiguousarray(norm, np.float32).reshape(-1)
+        t2p = None
+        if t2 is not None:
+            t2 = np.ascontiguousarray(t2, np.float32).reshape(-1)
+            t2p = t2.ctypes.data
+        rec = None if records is None else np.ascontiguousarray(records, np.int32).reshape(-1)
+        xD = np.empty(g.nvox, np.float32)
+        xT = np.empty(g.nvox, np.float32) if params.mode >= 2 else None
+        _check(self.h, lib().gfr_osem_run(self.h, C.byref(g.c()), C.byref(params), d2.ctypes.data,
+                                          t2p, norm.ctypes.data,
+                                          rec.ctypes.data if rec is not None else None,
+                                          0 if rec is None else rec.size // 2, xD.ctypes.data,
+                                          xT.ctypes.data if xT is not None else None))
+        return xD, xT
+
     def jbf3d(self, g, inp, guide, max_ref, r=4, sigma3d=0.75, sigmai=0.025, out=None):
         """jointbilateralfilter3D (GFIRST_GPU_Kernels.cu:508-544)."""
         i, _ = self._dev(inp)

@@ -422,6 +422,27 @@ int gfr_back_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_corr,
   return GFR_OK;
 }
 
+int gfr_sensitivity(gfr_ctx* ctx, const gfr_geometry* g, const uint8_t* d_mask, double* d_sens,
+                    int subset, int nsub, void* stream) {
+  GFR_CHECK_CTX(ctx);
+  GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, d_mask, "d_mask");
+  GFR_NONNULL(ctx, d_sens, "d_sens");
+  if (bad_subset(ctx, gd, subset, nsub)) return GFR_EINVAL;
+  hipStream_t st = stream_of(ctx, stream);
+  const size_t bytes = packed_bytes(gd->dev, subset, nsub);
+  hipError_t e = hipSuccess;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    float2* c2 = (float2*)scratch_acquire(ctx, ctx->pack, bytes, st, &e);
+    if (c2) e = launch_pack(gd->dev, nullptr, d_mask, c2, subset, nsub, st);
+    if (c2 && e == hipSuccess) e = launch_sensitivity(gd->dev, c2, d_sens, subset, nsub, st);
+    if (c2) scratch_release(ctx->pack, st);
+  }
+  if (e != hipSuccess) return hip_err(ctx, e, "sensitivity");
+  return GFR_OK;
+}
+
 int gfr_update(gfr_ctx* ctx, const gfr_geometry* g, float psf_bw, const float* d_numer,
                const float* d_denom, float* d_x, float* d_work, void* stream) {
   GFR_CHECK_CTX(ctx);

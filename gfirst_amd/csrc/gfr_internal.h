@@ -174,6 +174,10 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
 hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* numer, float* denom,
                               int subset, int nsub, const int* dsub, const Tune& tune,
                               hipStream_t st);
+// Sensitivity of a subset: the backprojector's denominator half in FP64
+// (packed rows carry ratio 1, so only calibration gaps apply).
+hipError_t launch_sensitivity(const DevGeom& G, const float2* corr2, double* sens, int subset,
+                              int nsub, hipStream_t st);
 // FP64 log-likelihood of the last forward pass (sum of its block partials).
 hipError_t launch_llh_sum(const DevGeom& G, const double* part, int subset, int nsub,
                           const int* dsub, int sg, double* out, hipStream_t st);

@@ -217,7 +217,8 @@ __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict
   }
 }
 
-// Full-layout ratio + byte mask -> packed {ratio, calib} rows of a subset.
+// Full-layout ratio (NULL: 1 everywhere) + byte mask -> packed {ratio,
+// calib} rows of a subset.
 __global__ void k_pack(DevGeom G, const float* __restrict__ corr, const uint8_t* __restrict__ mask,
                        float2* __restrict__ c2, int subset, int nsub) {
   const int r = blockIdx.x * kWave + threadIdx.x;
@@ -225,7 +226,7 @@ __global__ void k_pack(DevGeom G, const float* __restrict__ corr, const uint8_t*
   const int a = blockIdx.z;
   if (r >= G.nrad || s >= G.nsinogs) return;
   const size_t bin = ((size_t)s * G.nang + subset + a * nsub) * G.nrad + r;
-  store_packed(G, c2, a, s, r, make_float2(corr[bin], mask[bin] ? 1.f : -1000.f));
+  store_packed(G, c2, a, s, r, make_float2(corr ? corr[bin] : 1.f, mask[bin] ? 1.f : -1000.f));
 }
 
 
@@ -283,10 +284,10 @@ __device__ __forceinline__ LaneView lane_view(const DevGeom& G, float xv, float 
 // entries of this lane's two bins (r0, r1) of a plane: rows(i0) selects
 // planes i0 .. i0 + ZB, pair(k, a0, a1) reads plane i0 + k, get2(ii, a0, a1)
 // any plane.
-template <int ZB, class F>
+template <int ZB, class F, class T>
 __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, float yr, int zb,
-                                                int nzs, float fzm, float fzhi, F& fetch,
-                                                float* n, float* d) {
+                                                int nzs, float fzm, float fzhi, F& fetch, T* n,
+                                                T* d) {
   // Shared z axis. Slice j samples plane position w_j = fl(fl(P + zb + j) + 0.5)
   // with P = yr * fzb exact inside the fma. When s0 = fl(P + zb) >= 1 and
   // w_{ZB-1} lie in one binade [2^E, 2^(E+1)), every exact sum P + zb + j
@@ -317,8 +318,8 @@ __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, fl
       // in range and not a gap (GFIRST_GPU_Kernels.cu:157)
       const bool ok =
           v.any && (zb + j < nzs) && wj >= fzm && wj < fzhi && !(B < 0.f) && !(A < 0.f);
-      n[j] += ok ? A : 0.f;
-      d[j] += ok ? B : 0.f;
+      n[j] += ok ? (T)A : (T)0;
+      d[j] += ok ? (T)B : (T)0;
     }
     return;
   }
@@ -375,8 +376,8 @@ __device__ __forceinline__ void accumulate_view(const LaneView& v, float fzb, fl
       A = 0.f;
       B = 0.f;
     }
-    n[j] += A;
-    d[j] += B;
+    n[j] += (T)A;
+    d[j] += (T)B;
   }
 }
 
@@ -606,10 +607,10 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
 // MINW > 0 asks the compiler for a register budget that keeps MINW waves per
 // SIMD resident (an autotuner candidate; more resident waves can also thrash
 // the L1).
-template <int ZB, int MINW>
+template <int ZB, int MINW, class T = float>
 __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_back_direct(DevGeom G, const float2* __restrict__ c2,
-                                                     unsigned c2_bytes, float* __restrict__ numer,
-                                                     float* __restrict__ denom, int subset,
+                                                     unsigned c2_bytes, T* __restrict__ numer,
+                                                     T* __restrict__ denom, int subset,
                                                      int nsub, const int* __restrict__ dsub) {
   if (dsub) subset = *dsub;
   const int nav = (G.nang - subset + nsub - 1) / nsub;
@@ -623,9 +624,9 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_back_direct(DevGeo
   const unsigned rowb = (unsigned)packed_row_len(G.nrad) * 8u;
   const unsigned nr = (unsigned)(G.nsinogs + 2 * G.nseg);
   const float fz0 = (float)zb, fzl = (float)(zb + ZB - 1);
-  float n[ZB], d[ZB];
+  T n[ZB], d[ZB];
 #pragma unroll
-  for (int j = 0; j < ZB; j++) n[j] = d[j] = 0.f;
+  for (int j = 0; j < ZB; j++) n[j] = d[j] = (T)0;
   for (int si = 0; si < G.nseg; si++) {
     const SegEntry se = G.seg[si];
     const float fzm = (float)se.zmin, fzhi = (float)(G.nzs - se.zmin);
@@ -649,7 +650,7 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_back_direct(DevGeo
     const int iz = zb + j;
     if (iz < G.nzs) {
       const size_t iV = ((size_t)iz * G.res + iy) * G.res + ix;
-      numer[iV] = n[j];
+      if (numer) numer[iV] = n[j];
       denom[iV] = d[j];
     }
   }
@@ -780,6 +781,17 @@ hipError_t launch_pack(const DevGeom& G, const float* corr, const uint8_t* mask,
                        int subset, int nsub, hipStream_t st) {
   dim3 grid((G.nrad + kWave - 1) / kWave, (G.nsinogs + 3) / 4, nav_of(G, subset, nsub));
   hipLaunchKernelGGL(k_pack, grid, dim3(kWave, 4), 0, st, G, corr, mask, c2, subset, nsub);
+  return hipGetLastError();
+}
+
+hipError_t launch_sensitivity(const DevGeom& G, const float2* corr2, double* sens, int subset,
+                              int nsub, hipStream_t st) {
+  const size_t bytes = packed_bytes(G, subset, nsub);
+  if (bytes >= 0xFFFFFFF0u) return hipErrorInvalidValue;
+  const unsigned nbx = (G.res + 15) / 16;
+  hipLaunchKernelGGL((k_back_direct<4, 0, double>), dim3(nbx, nbx, (G.nzs + 3) / 4), dim3(256), 0,
+                     st, G, corr2, (unsigned)bytes, (double*)nullptr, sens, subset, nsub,
+                     (const int*)nullptr);
   return hipGetLastError();
 }
 

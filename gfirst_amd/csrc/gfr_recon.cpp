@@ -477,6 +477,22 @@ int gfr_recon_phase_times(gfr_recon* r, double* ms, long long* counts) {
   return GFR_OK;
 }
 
+int gfr_osem_run(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params* p, const float* d2,
+                 const float* t2, const float* norm, const int32_t* rec, long long nrec,
+                 float* xD, float* xT) {
+  if (!ctx) return GFR_EINVAL;
+  if (!xD || (p && p->mode >= 2 && !xT))
+    return set_err(ctx, GFR_EINVAL, "gfr_osem_run: output images required");
+  gfr_recon* r = nullptr;
+  int rc = gfr_recon_create(ctx, g, p, d2, t2, norm, rec, nrec, &r);
+  if (rc != GFR_OK) return rc;
+  rc = gfr_recon_run(r);
+  if (rc == GFR_OK) rc = gfr_recon_get(r, GFR_OUT_XD, xD);
+  if (rc == GFR_OK && p->mode >= 2) rc = gfr_recon_get(r, GFR_OUT_XT, xT);
+  gfr_recon_destroy(r);
+  return rc;
+}
+
 int gfr_recon_destroy(gfr_recon* r) {
   if (!r) return GFR_EINVAL;
   if (r->st) (void)hipStreamSynchronize(r->st);

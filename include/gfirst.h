@@ -184,6 +184,12 @@ int gfr_back_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_corr,
 /* Image update (DRV:482-502; GPU:173-188; CPU:165-187): R = DENOM<0.001 ? 1 :
  * NUMER/DENOM; F = Gz(Gxy_psf_bw(R)); x *= F; slice 0 := slice 1 and slice
  * NZS-1 := slice NZS-2. d_work is NVOX floats of scratch. In place on d_x. */
+/* Sensitivity image of a subset: the calibration half of BACKWPROJ
+ * (GPU:112-160 with CALIB_PART, DRV:447-467; ratio taken as 1, so only the
+ * calibration and segment-edge gaps apply), accumulated over views in FP64.
+ * d_sens: NVOX doubles, overwritten. */
+int gfr_sensitivity(gfr_ctx* ctx, const gfr_geometry* g, const uint8_t* d_mask, double* d_sens,
+                    int subset, int nsub, void* stream);
 int gfr_update(gfr_ctx* ctx, const gfr_geometry* g, float psf_bw, const float* d_numer,
                const float* d_denom, float* d_x, float* d_work, void* stream);
 
@@ -250,6 +256,11 @@ long long gfr_vlor_bin_hits(const gfr_geometry* g, const float* hits, long long 
 int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params* p,
                      const float* d2, const float* t2, const float* norm, const int32_t* rec,
                      long long nrec, gfr_recon** out);
+/* The whole of osem3d in one call (create, run, copy the images to the
+ * host, destroy): xD (and xT in modes 2-3) receive NVOX floats. */
+int gfr_osem_run(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params* p, const float* d2,
+                 const float* t2, const float* norm, const int32_t* rec, long long nrec,
+                 float* xD, float* xT);
 /* Device bytes a reconstruction with these parameters holds. */
 long long gfr_recon_footprint(const gfr_geometry* g, const gfr_recon_params* p);
 /* One (type, subset) pass; returns 1 when the reconstruction is complete. */

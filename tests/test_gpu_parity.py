@@ -159,6 +159,38 @@ def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg, lds, bw):
 
 
 @pytest.mark.parametrize("g", [SMALL, C0], ids=["small", "c0"])
+def test_sensitivity_fp64(gctx, orc, g):
+    """Calibration half of the backprojection in FP64: with every bin valid
+    each voxel counts the views whose plane range it falls in (exact
+    integers, equal to the FP32 oracle's DENOM for ratio 1); with a random
+    mask it is the oracle's DENOM up to FP32-vs-FP64 accumulation."""
+    nd = g.ndata
+    ones = np.ones(nd, np.float32)
+    for s, S in ((0, 1), (2, 4)):
+        sens = gctx.sensitivity(g, np.ones(nd, np.uint8), s, S)
+        _, de = orc.back(g, ones, ones, s, S)
+        assert sens.dtype == np.float64
+        assert np.array_equal(sens, de.astype(np.float64)), "all-valid sensitivity"
+        assert np.all(sens == np.round(sens))
+    rng = np.random.default_rng(33)
+    valid = rng.uniform(size=nd) < 0.9
+    sens = gctx.sensitivity(g, valid.astype(np.uint8), 1, 4)
+    _, de = orc.back(g, ones, np.where(valid, 1, -1000).astype(np.float32), 1, 4)
+    _, de64 = orc.back(g, ones.astype(np.float64), np.where(valid, 1.0, -1000.0), 1, 4, fp64=True)
+    close(sens, de, rtol=1e-5, atol=1e-6, what="sensitivity vs FP32 oracle")
+    close(sens, de64, rtol=1e-5, atol=1e-6, what="sensitivity vs FP64 oracle")
+
+
+def test_osem_run_one_call(gctx, orc):
+    g = SMALL
+    d2, t2, norm, _ = make_study(g, 41, 2)
+    p = ga.recon_params(mode=2, niter=1, nsub=3)
+    xD, xT = gctx.osem_run(g, p, d2, t2, norm)
+    exD, exT, _, _ = orc.osem(g, orc.params(mode=2, niter=1, nsub=3), d2, t2, norm)
+    assert np.array_equal(xD, exD) and np.array_equal(xT, exT)
+
+
+@pytest.mark.parametrize("g", [SMALL, C0], ids=["small", "c0"])
 def test_update(gctx, orc, g):
     rng = np.random.default_rng(9)
     num = rng.uniform(0, 50, g.nvox).astype(np.float32)
