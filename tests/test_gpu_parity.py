@@ -160,18 +160,18 @@ def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg, lds, bw):
 
 @pytest.mark.parametrize("g", [SMALL, C0], ids=["small", "c0"])
 def test_sensitivity_fp64(gctx, orc, g):
-    """Calibration half of the backprojection in FP64: with every bin valid
-    each voxel counts the views whose plane range it falls in (exact
-    integers, equal to the FP32 oracle's DENOM for ratio 1); with a random
-    mask it is the oracle's DENOM up to FP32-vs-FP64 accumulation."""
+    """Calibration half of the backprojection in FP64: the oracle's DENOM for
+    a ratio of 1 up to FP32-vs-FP64 accumulation (most voxels count whole
+    views exactly; segment-edge rows add fractions)."""
     nd = g.ndata
     ones = np.ones(nd, np.float32)
     for s, S in ((0, 1), (2, 4)):
         sens = gctx.sensitivity(g, np.ones(nd, np.uint8), s, S)
         _, de = orc.back(g, ones, ones, s, S)
         assert sens.dtype == np.float64
-        assert np.array_equal(sens, de.astype(np.float64)), "all-valid sensitivity"
-        assert np.all(sens == np.round(sens))
+        close(sens, de, rtol=1e-6, atol=1e-6, what="all-valid sensitivity S=%d" % S)
+        whole = de == np.round(de)
+        assert whole.mean() > 0.5 and np.array_equal(sens[whole], de[whole].astype(np.float64))
     rng = np.random.default_rng(33)
     valid = rng.uniform(size=nd) < 0.9
     sens = gctx.sensitivity(g, valid.astype(np.uint8), 1, 4)
