@@ -53,6 +53,7 @@ struct gfr_server {
   gfr_ctx* ctx = nullptr;
   int max_active = 0, inflight = 2;
   int creating = 0;       // sessions being created by admitters
+  long long reserved = 0; // budget bound of the sessions being created
   int retiring = 0;       // sessions being read back and freed
   bool mem_wait = false;  // a request did not fit: admit nothing until a retirement
   std::chrono::steady_clock::time_point t0;
@@ -95,6 +96,21 @@ void fail_req(gfr_server* s, const std::shared_ptr<Req>& q, int code, const std:
   s->n_failed++;
 }
 
+// Upper bound of the device bytes a request's session holds: the session
+// plan with both backgrounds, plus the VLOR lists in mode 3 (per record:
+// two indices, two event values, up to two bins with list offsets and two
+// list entries; BLANKT per bin).
+long long request_bound(const std::shared_ptr<Req>& q) {
+  const long long f = gfr_recon_footprint(&q->g, &q->p);
+  if (f < 0) return 0;  // invalid parameters: creation reports them
+  long long v = 0;
+  if (q->p.mode == 3) {
+    gfr_geometry_info inf{};
+    if (gfr_geometry_check(&q->g, &inf) == GFR_OK) v = 48 * q->nrec + 4 * inf.ndata + 64;
+  }
+  return f + v;
+}
+
 // Creates sessions for pending requests in arrival order as slots and
 // budget allow (several admitters run at once, so one large upload does not
 // hold back the requests behind it); a request that cannot fit even on an
@@ -111,14 +127,32 @@ void admitter(gfr_server* s) {
       continue;
     }
     std::shared_ptr<Req> q = s->pending.front();
+    // Budget decisions are taken here, in arrival order: the head is started
+    // only if an upper bound of its device bytes fits next to what is in use
+    // and what the sessions being created may take.
+    const long long bound = request_bound(q);
+    size_t budget = 0, in_use = 0;
+    {
+      std::lock_guard<std::mutex> lc(s->ctx->mu);
+      budget = s->ctx->budget;
+      in_use = s->ctx->in_use;
+    }
+    const bool idle = s->active.empty() && s->retire.empty() && s->creating == 0 &&
+                      s->retiring == 0;
+    if (budget && (long long)in_use + s->reserved + bound > (long long)budget && !idle) {
+      s->mem_wait = true;
+      continue;
+    }
     s->pending.pop_front();
     s->creating++;
+    s->reserved += bound;
     lk.unlock();
     gfr_recon* r = nullptr;
     int rc = gfr_recon_create(s->ctx, &q->g, &q->p, q->d2, q->t2, q->norm, q->rec, q->nrec, &r);
     std::string msg = rc == GFR_OK ? std::string() : std::string(gfr_last_error(s->ctx));
     lk.lock();
     s->creating--;
+    s->reserved -= bound;
     if (rc == GFR_ENOMEM &&
         !(s->active.empty() && s->retire.empty() && s->creating == 0 && s->retiring == 0)) {
       // memory frees up when running sessions retire: back into the queue in
