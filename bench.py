@@ -138,8 +138,7 @@ def run_c1(args):
     r = ctx.recon(g, p, d2, None, norm)
     pb = r.get(ga.OUT_PROJ_BLANK)
     tune = ctx.tuning(g, args.nsub)
-    log("autotuned tiles: back slices/thread=%d, z-sum sinograms/wave=%d (measured=%s)"
-        % tune[:3])
+    log("autotuned launch shapes: %s" % tune)
     log("reconstruction resident on device (%.2f GB)" % (ctx.bytes_in_use() / 1e9))
     for _ in range(args.warmup):
         r.step()
@@ -189,14 +188,12 @@ def run_c1(args):
         "roofline": {"kernel": "k_forward" if dom == "forward" else "k_back", "bound": "hbm",
                      "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 5),
-                     "traffic": measured_traffic(dom, tune[0], storage, tune[5]),
+                     "traffic": measured_traffic(dom, tune["back_zb"], storage, tune["back_lds"]),
                      "alg_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 4),
                      "fp32_tflops": round(flops, 2),
                      "fp32_frac": round(flops / PEAK_FP32_TFLOPS, 4)},
         "loglik_last_pass": ll,
-        "tiles": {"back_slices_per_thread": tune[0], "zsum_sinograms_per_wave": tune[1],
-                  "back_lds_window": tune[5], "back_waves_per_simd": tune[6],
-                  "autotuned": tune[2]},
+        "tiles": tune,
     }
     r.close()
     ctx.close()

@@ -39,6 +39,12 @@ class GfrReconParams(C.Structure):
                 ("storage", C.c_int)]
 
 
+class GfrTuning(C.Structure):
+    _fields_ = [("back_zb", C.c_int), ("back_lds", C.c_int), ("back_waves", C.c_int),
+                ("zsum_sg", C.c_int), ("rot_z", C.c_int), ("ms_back", C.c_float),
+                ("ms_zsum", C.c_float)]
+
+
 class GfrBatchParams(C.Structure):
     _fields_ = [("storage", C.c_int), ("max_frames", C.c_int), ("min_nsub", C.c_int),
                 ("lanes", C.c_int), ("psf_fw", C.c_float), ("psf_bw", C.c_float)]
@@ -91,9 +97,8 @@ SIGNATURES = {
     "gfr_memcpy_h2d": (_I, [_V, _V, _V, _SZ, _V]),
     "gfr_memcpy_d2h": (_I, [_V, _V, _V, _SZ, _V]),
     "gfr_zero": (_I, [_V, _V, _SZ, _V]),
-    "gfr_tuning_get": (_I, [_V, _G, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I),
-                            C.POINTER(_I), C.POINTER(_F), C.POINTER(_F)]),
-    "gfr_tuning_set": (_I, [_V, _G, _I, _I, _I, _I, _I]),
+    "gfr_tuning_get": (_I, [_V, _G, _I, C.POINTER(GfrTuning)]),
+    "gfr_tuning_set": (_I, [_V, _G, _I, C.POINTER(GfrTuning)]),
     "gfr_fill_cylinder": (_I, [_V, _G, _F, _V, _V]),
     "gfr_blur_xy": (_I, [_V, _G, _F, _V, _V, _V]),
     "gfr_forward_project": (_I, [_V, _G, _V, _V, _I, _I, _V]),

@@ -114,18 +114,16 @@ class Context:
 
     # -- tile shapes of the projectors -----------------------------------------
     def tuning(self, g, nsub):
-        """(back_zb, zsum_sg, measured, ms_back, ms_zsum, back_lds, back_waves) for
-        (geometry, nsub)."""
-        zb, sg, ld, bw = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        mb, mz = C.c_float(), C.c_float()
-        rc = _check(self.h, lib().gfr_tuning_get(self.h, C.byref(g.c()), nsub, C.byref(zb),
-                                                 C.byref(sg), C.byref(ld), C.byref(bw),
-                                                 C.byref(mb), C.byref(mz)))
-        return zb.value, sg.value, bool(rc), mb.value, mz.value, bool(ld.value), bw.value
+        """Launch shapes for (geometry, nsub) as a dict (+ "measured")."""
+        t = _lib.GfrTuning()
+        rc = _check(self.h, lib().gfr_tuning_get(self.h, C.byref(g.c()), nsub, C.byref(t)))
+        d = {k: getattr(t, k) for k, _ in t._fields_}
+        d["measured"] = bool(rc)
+        return d
 
-    def set_tuning(self, g, nsub, back_zb, zsum_sg, back_lds=False, back_waves=0):
-        _check(self.h, lib().gfr_tuning_set(self.h, C.byref(g.c()), nsub, back_zb, zsum_sg,
-                                            1 if back_lds else 0, back_waves))
+    def set_tuning(self, g, nsub, back_zb=8, zsum_sg=8, back_lds=False, back_waves=0, rot_z=4):
+        t = _lib.GfrTuning(back_zb, 1 if back_lds else 0, back_waves, zsum_sg, rot_z, 0.0, 0.0)
+        _check(self.h, lib().gfr_tuning_set(self.h, C.byref(g.c()), nsub, C.byref(t)))
 
     # -- operators (one per reference wrapper) ---------------------------------
     def fill_cylinder(self, g, val, out=None):

@@ -544,37 +544,43 @@ int gfr_vlor_destroy(gfr_vlor* v) {
 
 extern "C" {
 
-int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int* back_zb, int* zsum_sg,
-                   int* back_lds, int* back_waves, float* ms_back, float* ms_zsum) {
+int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, gfr_tuning* out) {
   GFR_CHECK_CTX(ctx);
   GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, out, "tuning");
   if (nsub < 1 || nsub > g->nang) return set_err(ctx, GFR_EINVAL, "nsub out of range");
   const Tune t = tune_for(ctx, gd, nsub);
-  if (back_zb) *back_zb = t.back_zb;
-  if (zsum_sg) *zsum_sg = t.zsum_sg;
-  if (back_lds) *back_lds = t.back_lds ? 1 : 0;
-  if (back_waves) *back_waves = t.back_waves;
-  if (ms_back) *ms_back = t.ms_back;
-  if (ms_zsum) *ms_zsum = t.ms_zsum;
+  out->back_zb = t.back_zb;
+  out->back_lds = t.back_lds ? 1 : 0;
+  out->back_waves = t.back_waves;
+  out->zsum_sg = t.zsum_sg;
+  out->rot_z = t.rot_z;
+  out->ms_back = t.ms_back;
+  out->ms_zsum = t.ms_zsum;
   return t.tuned ? 1 : 0;
 }
 
-int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int back_zb, int zsum_sg,
-                   int back_lds, int back_waves) {
+int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, const gfr_tuning* in) {
   GFR_CHECK_CTX(ctx);
   GFR_GEOM(ctx, g, gd);
+  GFR_NONNULL(ctx, in, "tuning");
   if (nsub < 1 || nsub > g->nang) return set_err(ctx, GFR_EINVAL, "nsub out of range");
   auto ok = [](int v) { return v == 4 || v == 8 || v == 16; };
-  if (!ok(back_zb) || !ok(zsum_sg)) return set_err(ctx, GFR_EINVAL, "tile sizes must be 4, 8 or 16");
-  if (back_lds != 0 && back_lds != 1) return set_err(ctx, GFR_EINVAL, "back_lds must be 0 or 1");
-  if (back_waves != 0 && !(back_waves == 6 && back_zb == 8 && !back_lds))
+  if (!ok(in->back_zb) || !ok(in->zsum_sg))
+    return set_err(ctx, GFR_EINVAL, "back_zb and zsum_sg must be 4, 8 or 16");
+  if (in->back_lds != 0 && in->back_lds != 1)
+    return set_err(ctx, GFR_EINVAL, "back_lds must be 0 or 1");
+  if (in->back_waves != 0 && !(in->back_waves == 6 && in->back_zb == 8 && !in->back_lds))
     return set_err(ctx, GFR_EINVAL, "back_waves: 0, or 6 with 8 slices and direct reads");
+  if (in->rot_z != 1 && in->rot_z != 4 && in->rot_z != 16)
+    return set_err(ctx, GFR_EINVAL, "rot_z must be 1, 4 or 16");
   std::lock_guard<std::mutex> lk(ctx->mu);
   Tune& t = gd->tune[nsub];
-  t.back_zb = back_zb;
-  t.zsum_sg = zsum_sg;
-  t.back_lds = back_lds != 0;
-  t.back_waves = back_waves;
+  t.back_zb = in->back_zb;
+  t.back_lds = in->back_lds != 0;
+  t.back_waves = in->back_waves;
+  t.zsum_sg = in->zsum_sg;
+  t.rot_z = in->rot_z;
   t.tuned = true;
   t.ms_back = t.ms_zsum = 0.f;
   return GFR_OK;

@@ -42,16 +42,16 @@ __device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, unsigned voff,
 
 // ---------------------------------------------------------------------------
 // k_rotate: one thread per (angle a of the chunk, sample l, bin r) and a run
-// of kRotZ planes: the sample's x/y position and weights do not depend on
-// the plane, so they are computed once and the planes' four texels are
-// gathered at fixed offsets. Logical block order (r-block, l-block,
-// plane-run, a) with XCD-contiguous ranges.
+// of RZ planes (autotuned): the sample's x/y position and weights do not
+// depend on the plane, so they are computed once per run and the planes'
+// four texels are gathered at fixed offsets. Logical block order (r-block,
+// l-block, plane-run, a) with XCD-contiguous ranges.
 // ---------------------------------------------------------------------------
-constexpr int kRotZ = 16;
-
+template <int RZ>
 __global__ __launch_bounds__(256) void k_rotate(DevGeom G, const float* __restrict__ img,
                                                 float* __restrict__ R, int subset, int nsub,
                                                 const int* __restrict__ dsub, int a0, int nl) {
+  constexpr int kRotZ = RZ;
   if (dsub) subset = *dsub;
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const unsigned nrb = (G.nrad + kWave - 1) / kWave, nlb = (nl + 3) / 4;
@@ -740,9 +740,17 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
   for (int a0 = 0; a0 < nav; a0 += chunk) {
     const int na = min(chunk, nav - a0);
     const unsigned rbytes = (unsigned)((size_t)na * per * sizeof(float));
-    const unsigned grot = nrb * ((nl + 3) / 4) * ((G.nzs + kRotZ - 1) / kRotZ) * na;
-    hipLaunchKernelGGL(k_rotate, dim3(grot), dim3(kWave, 4), 0, st, G, img, R, subset, nsub, dsub,
-                       a0, nl);
+    const int rz = tune.rot_z == 1 || tune.rot_z == 4 ? tune.rot_z : 16;
+    const unsigned grot = nrb * ((nl + 3) / 4) * ((G.nzs + rz - 1) / rz) * na;
+    if (rz == 1)
+      hipLaunchKernelGGL(k_rotate<1>, dim3(grot), dim3(kWave, 4), 0, st, G, img, R, subset, nsub,
+                         dsub, a0, nl);
+    else if (rz == 4)
+      hipLaunchKernelGGL(k_rotate<4>, dim3(grot), dim3(kWave, 4), 0, st, G, img, R, subset, nsub,
+                         dsub, a0, nl);
+    else
+      hipLaunchKernelGGL(k_rotate<16>, dim3(grot), dim3(kWave, 4), 0, st, G, img, R, subset, nsub,
+                         dsub, a0, nl);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     switch (tune.zsum_sg) {
@@ -871,6 +879,18 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
     if (ms < bz) {
       bz = ms;
       t.zsum_sg = c;
+    }
+  }
+  const int rcand[3] = {1, 4, 16};  // planes per rotation thread
+  for (int c : rcand) {
+    Tune tt = t;
+    tt.rot_z = c;
+    const float ms = time_it([&] {
+      return launch_forward(G, img, R, r_floats, fo, 0, nsub, nullptr, tt, st);
+    });
+    if (ms < bz) {
+      bz = ms;
+      t.rot_z = c;
     }
   }
   // backprojector: (kind, slices, register budget)

@@ -123,21 +123,23 @@ int gfr_memcpy_d2h(gfr_ctx* ctx, void* dst, const void* src, size_t bytes, void*
 /* Zero fill with a kernel (CleanMemory, GPU:207-219). */
 int gfr_zero(gfr_ctx* ctx, void* dptr, size_t bytes, void* stream);
 
-/* Tile shapes of the projectors for (geometry, nsub): back_zb = slices per
- * backprojector thread (4, 8, 16), back_lds = 1 when the backprojector stages
- * each view's rows for a 16x16-column tile in LDS (0: direct cached reads),
- * back_waves = 6 for the direct 8-slice kernel compiled to keep 6 waves per
- * SIMD resident (0: the compiler's register choice), zsum_sg = sinograms per
- * forward z-sum wave (4, 8, 16). A reconstruction session measures them once
- * per context (env GFR_AUTOTUNE=0 keeps the defaults 8/8/direct/0);
- * per-operation calls use the context's current choice. Results do not depend
- * on the shapes. No counterpart in the reference (its launch shapes are
- * fixed, GPU:105-107, 166-171). get returns 1 if the shapes were measured, 0
- * if defaults. */
-int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int* back_zb, int* zsum_sg,
-                   int* back_lds, int* back_waves, float* ms_back, float* ms_zsum);
-int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, int back_zb, int zsum_sg,
-                   int back_lds, int back_waves);
+/* Launch shapes of the projectors for (geometry, nsub). Results do not
+ * depend on them. A reconstruction session measures them once per context
+ * (env GFR_AUTOTUNE=0 keeps the defaults); per-operation calls use the
+ * context's current choice. No counterpart in the reference (its launch
+ * shapes are fixed, GPU:105-107, 166-171). */
+typedef struct gfr_tuning {
+  int back_zb;    /* slices per backprojector thread: 4, 8 or 16                 */
+  int back_lds;   /* 1: per-block LDS window of each view's rows; 0: cached reads */
+  int back_waves; /* 6: 8-slice direct kernel built for 6 waves/SIMD; else 0     */
+  int zsum_sg;    /* sinograms per forward z-sum wave: 4, 8 or 16                */
+  int rot_z;      /* planes per forward rotation thread: 1, 4 or 16             */
+  float ms_back;  /* measured time of the chosen backprojector (0 if not measured) */
+  float ms_zsum;  /* measured time of the chosen forward projector              */
+} gfr_tuning;
+/* Returns 1 if the shapes were measured, 0 if defaults. */
+int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, gfr_tuning* t);
+int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, const gfr_tuning* t);
 
 /* ---------------- per-operation entry points (stream ordered) ---------------- */
 
