@@ -104,24 +104,14 @@ GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g) {
   const size_t off_cos = n_ang * sizeof(float);
   const size_t off_seg = ((2 * n_ang * sizeof(float)) + 15) & ~size_t(15);
   const size_t off_sino = (off_seg + n_seg * sizeof(SegEntry) + 15) & ~size_t(15);
-  // z tables: indices, weights, then the three group tables (16-byte aligned)
-  auto al16 = [](size_t x) { return (x + 15) & ~size_t(15); };
-  const size_t off_zi = al16(off_sino + n_sino * sizeof(int2));
-  const size_t off_za = al16(off_zi + h.zidx.size() * sizeof(uint32_t));
-  size_t off_zg[3];
-  off_zg[0] = al16(off_za + h.zal.size() * sizeof(float));
-  off_zg[1] = al16(off_zg[0] + h.zgrp[0].size() * sizeof(uint32_t));
-  off_zg[2] = al16(off_zg[1] + h.zgrp[1].size() * sizeof(uint32_t));
-  const size_t total = al16(off_zg[2] + h.zgrp[2].size() * sizeof(uint32_t));
+  const size_t off_zt = (off_sino + n_sino * sizeof(int2) + 15) & ~size_t(15);
+  const size_t total = off_zt + h.ztab.size() * sizeof(uint2);
   std::vector<char> blob(total, 0);
   memcpy(blob.data(), h.sin_t.data(), n_ang * sizeof(float));
   memcpy(blob.data() + off_cos, h.cos_t.data(), n_ang * sizeof(float));
   memcpy(blob.data() + off_seg, h.seg.data(), n_seg * sizeof(SegEntry));
   memcpy(blob.data() + off_sino, h.sino.data(), n_sino * sizeof(int2));
-  memcpy(blob.data() + off_zi, h.zidx.data(), h.zidx.size() * sizeof(uint32_t));
-  memcpy(blob.data() + off_za, h.zal.data(), h.zal.size() * sizeof(float));
-  for (int k = 0; k < 3; k++)
-    memcpy(blob.data() + off_zg[k], h.zgrp[k].data(), h.zgrp[k].size() * sizeof(uint32_t));
+  memcpy(blob.data() + off_zt, h.ztab.data(), h.ztab.size() * sizeof(uint2));
   hipError_t e = hipMalloc(&gd->block, total);
   if (e == hipSuccess) e = hipMemcpy(gd->block, blob.data(), total, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -152,9 +142,7 @@ GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g) {
   d.cos_t = (const float*)(base + off_cos);
   d.seg = (const SegEntry*)(base + off_seg);
   d.sino = (const int2*)(base + off_sino);
-  d.zidx = (const uint32_t*)(base + off_zi);
-  d.zal = (const float*)(base + off_za);
-  for (int k = 0; k < 3; k++) d.zgrp[k] = (const uint32_t*)(base + off_zg[k]);
+  d.ztab = (const uint2*)(base + off_zt);
   ctx->geoms[key] = gd;
   return gd;
 }

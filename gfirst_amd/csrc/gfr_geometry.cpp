@@ -91,8 +91,7 @@ int derive_geometry(const gfr_geometry* g, GeomHost* out, std::string* err, bool
   if (g->nzs > 0xFFFF) return fail("geometry: nzs too large");
   if (!with_ztab) return GFR_OK;
   const int nl = h.info.lambda1 - h.info.lambda0 + 1;
-  h.zidx.resize((size_t)acc * nl);
-  h.zal.resize((size_t)acc * nl);
+  h.ztab.resize((size_t)acc * nl);
   for (int s = 0; s < acc; s++) {
     const float dz = h.seg[h.sino[s].x].dzf;
     const float zc = (float)h.sino[s].y + 0.5f;
@@ -104,23 +103,10 @@ int derive_geometry(const gfr_geometry* g, GeomHost* out, std::string* err, bool
       const int i = (int)f;
       const int z0 = i < 0 ? 0 : (i > g->nzs - 1 ? g->nzs - 1 : i);
       const int z1 = i + 1 < 0 ? 0 : (i + 1 > g->nzs - 1 ? g->nzs - 1 : i + 1);
-      h.zidx[(size_t)li * acc + s] = (uint32_t)z0 | ((uint32_t)z1 << 16);
-      h.zal[(size_t)li * acc + s] = a;
+      uint32_t ab;
+      memcpy(&ab, &a, 4);
+      h.ztab[(size_t)li * acc + s] = make_uint2((uint32_t)z0 | ((uint32_t)z1 << 16), ab);
     }
-  }
-  const int sgs[3] = {4, 8, 16};
-  for (int k = 0; k < 3; k++) {
-    const int sg = sgs[k], ng = (acc + sg - 1) / sg;
-    h.zgrp[k].assign((size_t)nl * ng, 0xFFFFFFFFu);
-    for (int li = 0; li < nl; li++)
-      for (int gi = 0; gi < ng; gi++) {
-        const int s0 = gi * sg, ns = std::min(sg, acc - s0);
-        const uint32_t* e = &h.zidx[(size_t)li * acc + s0];
-        const uint32_t z00 = e[0] & 0xFFFFu;
-        bool reg = true;
-        for (int i = 0; i < ns && reg; i++) reg = e[i] == ((z00 + i) | ((z00 + i + 1) << 16));
-        if (reg) h.zgrp[k][(size_t)li * ng + gi] = z00;
-      }
   }
   return GFR_OK;
 }

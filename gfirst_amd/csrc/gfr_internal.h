@@ -36,15 +36,9 @@ struct DevGeom {
   const float* cos_t;     // [nang]
   const SegEntry* seg;    // [nseg]
   const int2* sino;       // [nsinogs]: (segment index, absolute plane)
-  // [l1-l0+1][nsinogs]: z interpolation of each LOR sample
-  // (GFIRST_GPU_Kernels.cu:88, 96 through the clamp filter): planes z0 | z1 << 16
-  // and the weight alpha_z
-  const uint32_t* zidx;
-  const float* zal;
-  // [3][l1-l0+1][groups of SG sinograms], SG = 4, 8, 16 (aligned groups): the
-  // first plane z00 when every sinogram i of the group blends planes z00 + i
-  // and z00 + i + 1 at that sample, else 0xFFFFFFFF
-  const uint32_t* zgrp[3];
+  // [l1-l0+1][nsinogs]: z interpolation of each LOR sample, x = z0 | z1 << 16,
+  // y = bits of alpha_z (GFIRST_GPU_Kernels.cu:88, 96 through the clamp filter)
+  const uint2* ztab;
 };
 
 struct Taps9 { float g[9]; };
@@ -70,9 +64,7 @@ struct GeomHost {
   std::vector<float> sin_t, cos_t;
   std::vector<SegEntry> seg;
   std::vector<int2> sino;
-  std::vector<uint32_t> zidx;
-  std::vector<float> zal;
-  std::vector<uint32_t> zgrp[3];
+  std::vector<uint2> ztab;
   float RADM, RESM, fxf, fxb;
 };
 

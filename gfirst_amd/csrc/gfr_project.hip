@@ -109,11 +109,10 @@ struct FwdArgs {
 // k_zsum: one wave per (angle a, SG consecutive sinograms, 128 radial bins);
 // lane l owns bins 2l and 2l+1 of each sinogram of the group, so a row load is
 // one contiguous 512-byte dwordx2 access. The z interpolation of a sample is
-// wave-uniform (scalar loads of the transposed z tables). When the group's
+// wave-uniform (scalar loads of the transposed z table). When the group's
 // samples at l blend planes z0, z0+1, ..., z0+SG (consecutive planes: the
-// usual case inside a segment; a precomputed per-group table says so), the
-// SG+1 rows are loaded once for the group; otherwise each LOR loads its own
-// two rows. Every LOR still sums its samples
+// usual case inside a segment), the SG+1 rows are loaded once for the group;
+// otherwise each LOR loads its own two rows. Every LOR still sums its samples
 // in l order with its own weights. Epilogue: ratio (GFIRST.cu:415-430) and the
 // FP64 Poisson log-likelihood partial of the block.
 // ---------------------------------------------------------------------------
@@ -145,35 +144,35 @@ __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict
     float acc0[SG], acc1[SG];
 #pragma unroll
     for (int i = 0; i < SG; i++) acc0[i] = acc1[i] = 0.f;
-    // group table of this SG: first plane of a regular group, else ~0
-    const uint32_t* zg = G.zgrp[SG == 4 ? 0 : (SG == 8 ? 1 : 2)];
-    const int ngrp = (G.nsinogs + SG - 1) / SG, gi = s0 / SG;
     unsigned soff = (unsigned)a * (unsigned)nl * planeb;
     for (int li = 0; li < nl; li++, soff += planeb) {
-      const size_t zrow = (size_t)li * G.nsinogs + s0;
-      const float* za = G.zal + zrow;
-      float al[SG];
+      const uint2* zt = G.ztab + (size_t)li * G.nsinogs + s0;
+      uint2 e[SG];
 #pragma unroll
-      for (int i = 0; i < SG; i++) al[i] = za[i < ns ? i : ns - 1];
-      const unsigned z00 = zg[(size_t)li * ngrp + gi];
-      if (z00 != 0xFFFFFFFFu) {
+      for (int i = 0; i < SG; i++) e[i] = zt[i < ns ? i : ns - 1];
+      const unsigned z00 = e[0].x & 0xFFFFu;
+      bool reg = true;
+#pragma unroll
+      for (int i = 0; i < SG; i++)
+        reg = reg && (e[i].x == ((z00 + i) | ((z00 + i + 1) << 16)));
+      if (reg) {
         float2 row[SG + 1];
 #pragma unroll
         for (int k = 0; k <= SG; k++) row[k] = ld_f2(rr, voff, soff + (z00 + k) * rowb);
 #pragma unroll
         for (int i = 0; i < SG; i++) {
-          acc0[i] += lerp1(row[i].x, row[i + 1].x, al[i]);
-          acc1[i] += lerp1(row[i].y, row[i + 1].y, al[i]);
+          const float al = __uint_as_float(e[i].y);
+          acc0[i] += lerp1(row[i].x, row[i + 1].x, al);
+          acc1[i] += lerp1(row[i].y, row[i + 1].y, al);
         }
       } else {
-        const uint32_t* zi = G.zidx + zrow;
 #pragma unroll
         for (int i = 0; i < SG; i++) {
-          const uint32_t e = zi[i < ns ? i : ns - 1];
-          const float2 v0 = ld_f2(rr, voff, soff + (e & 0xFFFFu) * rowb);
-          const float2 v1 = ld_f2(rr, voff, soff + (e >> 16) * rowb);
-          acc0[i] += lerp1(v0.x, v1.x, al[i]);
-          acc1[i] += lerp1(v0.y, v1.y, al[i]);
+          const float al = __uint_as_float(e[i].y);
+          const float2 v0 = ld_f2(rr, voff, soff + (e[i].x & 0xFFFFu) * rowb);
+          const float2 v1 = ld_f2(rr, voff, soff + (e[i].x >> 16) * rowb);
+          acc0[i] += lerp1(v0.x, v1.x, al);
+          acc1[i] += lerp1(v0.y, v1.y, al);
         }
       }
     }
