@@ -145,11 +145,19 @@ __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict
 #pragma unroll
     for (int i = 0; i < SG; i++) acc0[i] = acc1[i] = 0.f;
     unsigned soff = (unsigned)a * (unsigned)nl * planeb;
+    // z-table entries of sample li, loaded one sample ahead
+    uint2 en[SG];
+#pragma unroll
+    for (int i = 0; i < SG; i++) en[i] = G.ztab[s0 + (i < ns ? i : ns - 1)];
     for (int li = 0; li < nl; li++, soff += planeb) {
-      const uint2* zt = G.ztab + (size_t)li * G.nsinogs + s0;
       uint2 e[SG];
 #pragma unroll
-      for (int i = 0; i < SG; i++) e[i] = zt[i < ns ? i : ns - 1];
+      for (int i = 0; i < SG; i++) e[i] = en[i];
+      if (li + 1 < nl) {
+        const uint2* zt = G.ztab + (size_t)(li + 1) * G.nsinogs + s0;
+#pragma unroll
+        for (int i = 0; i < SG; i++) en[i] = zt[i < ns ? i : ns - 1];
+      }
       const unsigned z00 = e[0].x & 0xFFFFu;
       bool reg = true;
 #pragma unroll
