@@ -122,7 +122,7 @@ class Context:
         return d
 
     def set_tuning(self, g, nsub, back_zb=8, zsum_sg=8, back_lds=False, back_waves=0, rot_z=4):
-        t = _lib.GfrTuning(back_zb, 1 if back_lds else 0, back_waves, zsum_sg, rot_z, 0.0, 0.0)
+        t = _lib.GfrTuning(back_zb, int(back_lds), back_waves, zsum_sg, rot_z, 0.0, 0.0)
         _check(self.h, lib().gfr_tuning_set(self.h, C.byref(g.c()), nsub, C.byref(t)))
 
     # -- operators (one per reference wrapper) ---------------------------------

@@ -539,7 +539,7 @@ int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, gfr_tuning* ou
   if (nsub < 1 || nsub > g->nang) return set_err(ctx, GFR_EINVAL, "nsub out of range");
   const Tune t = tune_for(ctx, gd, nsub);
   out->back_zb = t.back_zb;
-  out->back_lds = t.back_lds ? 1 : 0;
+  out->back_lds = t.back_lds;
   out->back_waves = t.back_waves;
   out->zsum_sg = t.zsum_sg;
   out->rot_z = t.rot_z;
@@ -565,7 +565,7 @@ int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, const gfr_tuni
   std::lock_guard<std::mutex> lk(ctx->mu);
   Tune& t = gd->tune[nsub];
   t.back_zb = in->back_zb;
-  t.back_lds = in->back_lds != 0;
+  t.back_lds = in->back_lds;
   t.back_waves = in->back_waves;
   t.zsum_sg = in->zsum_sg;
   t.rot_z = in->rot_z;

@@ -48,7 +48,7 @@ struct Taps9 { float g[9]; };
 struct Tune {
   int back_zb = 8;   // slices per backprojector thread: 4, 8 or 16
   int zsum_sg = 8;   // consecutive sinograms per z-sum wave: 4, 8 or 16
-  bool back_lds = false;  // backprojector reads its rows through an LDS window
+  int back_lds = 0;  // 1: backprojector reads its rows through an LDS window
   int back_waves = 0;     // register budget for this many waves per SIMD (0: compiler's)
   int rot_z = 4;          // planes per rotation thread: 1, 4 or 16
   bool tuned = false;

@@ -62,8 +62,11 @@ __global__ __launch_bounds__(256) void k_rotate(DevGeom G, const float* __restri
   q /= nlb;
   const int zb = (int)(q % nzb) * kRotZ;
   const int a = (int)(q / nzb);
-  const int r = rb * kWave + threadIdx.x;
-  const int li = lb * 4 + threadIdx.y;
+  // a wave covers 16 bins x 4 samples: its sample points form a compact
+  // patch of the image, so each gather touches few cache lines
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int r = rb * kWave + (tid >> 6) * 16 + (lane & 15);
+  const int li = lb * 4 + (lane >> 4);
   if (r >= G.nrad || li >= nl || a0 + a >= (G.nang - subset + nsub - 1) / nsub) return;
   const int it = subset + (a0 + a) * nsub;
   const float st = G.sin_t[it], ct = G.cos_t[it];
@@ -106,45 +109,52 @@ struct FwdArgs {
 };
 
 // ---------------------------------------------------------------------------
-// k_zsum: one wave per (angle a, SG consecutive sinograms, 128 radial bins);
-// lane l owns bins 2l and 2l+1 of each sinogram of the group, so a row load is
-// one contiguous 512-byte dwordx2 access. The z interpolation of a sample is
-// wave-uniform (scalar loads of the transposed z table). When the group's
-// samples at l blend planes z0, z0+1, ..., z0+SG (consecutive planes: the
-// usual case inside a segment), the SG+1 rows are loaded once for the group;
-// otherwise each LOR loads its own two rows. Every LOR still sums its samples
-// in l order with its own weights. Epilogue: ratio (GFIRST.cu:415-430) and the
-// FP64 Poisson log-likelihood partial of the block.
+// k_zsum: one wave per (pair of angles, SG consecutive sinograms, 128 radial
+// bins). Half-wave h takes angle 2p + h of the chunk and lane l of the half
+// owns bins 4l .. 4l+3, so a row load is one 16-byte access per lane and one
+// instruction fetches a row of both angles (the z interpolation depends on
+// the sinogram and the sample, not on the angle). The z interpolation of a
+// sample is wave-uniform (scalar loads of the transposed z table). When the
+// group's samples at l blend planes z0, z0+1, ..., z0+SG (consecutive planes:
+// the usual case inside a segment), the SG+1 rows are loaded once for the
+// group; otherwise each LOR loads its own two rows. Every LOR still sums its
+// samples in l order with its own weights. Epilogue: ratio
+// (GFIRST.cu:415-430) and the FP64 Poisson log-likelihood partial of the
+// block.
 // ---------------------------------------------------------------------------
 template <int S, bool RATIO, bool LLH, int SG>
 __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict__ R,
                                               unsigned r_bytes, FwdArgs o, int subset, int nsub,
-                                              const int* __restrict__ dsub, int a0, int nl,
-                                              unsigned llh_off) {
+                                              const int* __restrict__ dsub, int a0, int na,
+                                              int nl, unsigned llh_off) {
   if (dsub) subset = *dsub;
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const unsigned nrb = (G.nrad + 127) / 128, ngb = (G.nsinogs + 4 * SG - 1) / (4 * SG);
   const unsigned rb = b % nrb;
   const unsigned q = b / nrb;
   const unsigned gb = q % ngb;
-  const int a = (int)(q / ngb);
+  const int ap = (int)(q / ngb);
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s0 = __builtin_amdgcn_readfirstlane((int)((gb * 4 + wv) * SG));
-  const int r = (int)rb * 128 + 2 * lane;
+  const int a = 2 * ap + (lane >> 5);  // angle of the chunk
+  const int r = (int)rb * 128 + 4 * (lane & 31);
+  const int nav = (G.nang - subset + nsub - 1) / nsub;
   const int ag = a0 + a;
   const int it = subset + ag * nsub;
-  if (ag >= (G.nang - subset + nsub - 1) / nsub) return;  // whole block: angle not in subset
+  if (a0 + 2 * ap >= nav) return;  // whole block: no angle of the pair in the subset
+  const bool aok = a < na && ag < nav;
   double ll = 0.0;
   if (s0 < G.nsinogs) {
     const int ns = min(SG, G.nsinogs - s0);
+    // Lanes of an angle past the chunk read past r_bytes: zeros, unused.
     const __amdgpu_buffer_rsrc_t rr = rsrc(R, r_bytes);
     const unsigned rowb = (unsigned)G.nrad * 4u;
     const unsigned planeb = rowb * (unsigned)G.nzs;
-    const unsigned voff = (unsigned)r * 4u;
-    float acc0[SG], acc1[SG];
+    const unsigned voff = (unsigned)a * (unsigned)nl * planeb + (unsigned)r * 4u;
+    float4 acc[SG];
 #pragma unroll
-    for (int i = 0; i < SG; i++) acc0[i] = acc1[i] = 0.f;
-    unsigned soff = (unsigned)a * (unsigned)nl * planeb;
+    for (int i = 0; i < SG; i++) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    unsigned soff = 0;
     // z-table entries of sample li, loaded one sample ahead
     uint2 en[SG];
 #pragma unroll
@@ -164,48 +174,55 @@ __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict
       for (int i = 0; i < SG; i++)
         reg = reg && (e[i].x == ((z00 + i) | ((z00 + i + 1) << 16)));
       if (reg) {
-        float2 row[SG + 1];
+        float4 row[SG + 1];
 #pragma unroll
-        for (int k = 0; k <= SG; k++) row[k] = ld_f2(rr, voff, soff + (z00 + k) * rowb);
+        for (int k = 0; k <= SG; k++) row[k] = ld_f4(rr, voff, soff + (z00 + k) * rowb);
 #pragma unroll
         for (int i = 0; i < SG; i++) {
           const float al = __uint_as_float(e[i].y);
-          acc0[i] += lerp1(row[i].x, row[i + 1].x, al);
-          acc1[i] += lerp1(row[i].y, row[i + 1].y, al);
+          acc[i].x += lerp1(row[i].x, row[i + 1].x, al);
+          acc[i].y += lerp1(row[i].y, row[i + 1].y, al);
+          acc[i].z += lerp1(row[i].z, row[i + 1].z, al);
+          acc[i].w += lerp1(row[i].w, row[i + 1].w, al);
         }
       } else {
 #pragma unroll
         for (int i = 0; i < SG; i++) {
           const float al = __uint_as_float(e[i].y);
-          const float2 v0 = ld_f2(rr, voff, soff + (e[i].x & 0xFFFFu) * rowb);
-          const float2 v1 = ld_f2(rr, voff, soff + (e[i].x >> 16) * rowb);
-          acc0[i] += lerp1(v0.x, v1.x, al);
-          acc1[i] += lerp1(v0.y, v1.y, al);
+          const float4 v0 = ld_f4(rr, voff, soff + (e[i].x & 0xFFFFu) * rowb);
+          const float4 v1 = ld_f4(rr, voff, soff + (e[i].x >> 16) * rowb);
+          acc[i].x += lerp1(v0.x, v1.x, al);
+          acc[i].y += lerp1(v0.y, v1.y, al);
+          acc[i].z += lerp1(v0.z, v1.z, al);
+          acc[i].w += lerp1(v0.w, v1.w, al);
         }
       }
     }
+    if (aok) {
 #pragma unroll
-    for (int i = 0; i < SG; i++) {
-      if (i >= ns) break;
-      const int s = s0 + i;
+      for (int i = 0; i < SG; i++) {
+        if (i >= ns) break;
+        const int s = s0 + i;
+        const float av[4] = {acc[i].x, acc[i].y, acc[i].z, acc[i].w};
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int rr2 = r + h;
-        if (rr2 >= G.nrad) break;
-        const float acc = h ? acc1[i] : acc0[i];
-        const size_t bin = ((size_t)s * G.nang + it) * G.nrad + rr2;
-        if (o.proj) o.proj[bin] = acc;
-        if constexpr (RATIO) {
-          const float yv = o.yscale * load_store<S>(o.y, bin);
-          float bv = o.bg ? o.yscale * load_store<S>(o.bg, bin) : 0.f;
-          if (bv < 0.f) bv = 0.f;
-          const float den = acc + bv;
-          const float c = (den >= 0.01f) ? yv / den : -1000.f;
-          if (o.corr) o.corr[bin] = c;
-          if (o.corr2)
-            store_packed(G, o.corr2, ag, s, rr2, make_float2(c, o.mask[bin] ? 1.f : -1000.f));
-          if constexpr (LLH) {
-            if (den > 0.f) ll += (double)yv * log((double)den) - (double)den;
+        for (int h = 0; h < 4; h++) {
+          const int rr2 = r + h;
+          if (rr2 >= G.nrad) break;
+          const float accv = av[h];
+          const size_t bin = ((size_t)s * G.nang + it) * G.nrad + rr2;
+          if (o.proj) o.proj[bin] = accv;
+          if constexpr (RATIO) {
+            const float yv = o.yscale * load_store<S>(o.y, bin);
+            float bv = o.bg ? o.yscale * load_store<S>(o.bg, bin) : 0.f;
+            if (bv < 0.f) bv = 0.f;
+            const float den = accv + bv;
+            const float c = (den >= 0.01f) ? yv / den : -1000.f;
+            if (o.corr) o.corr[bin] = c;
+            if (o.corr2)
+              store_packed(G, o.corr2, ag, s, rr2, make_float2(c, o.mask[bin] ? 1.f : -1000.f));
+            if constexpr (LLH) {
+              if (den > 0.f) ll += (double)yv * log((double)den) - (double)den;
+            }
           }
         }
       }
@@ -673,8 +690,13 @@ size_t rotate_floats_per_angle(const DevGeom& G) {
   return (size_t)(G.l1 - G.l0 + 1) * G.nzs * G.nrad;
 }
 
+// z-sum blocks (= log-likelihood partials) per pair of angles, and in all
+int zsum_blocks_per_pair(const DevGeom& G, int sg) {
+  return ((G.nrad + 127) / 128) * ((G.nsinogs + 4 * sg - 1) / (4 * sg));
+}
+
 int forward_llh_blocks(const DevGeom& G, int subset, int nsub, int sg) {
-  return ((G.nrad + 127) / 128) * ((G.nsinogs + 4 * sg - 1) / (4 * sg)) * nav_of(G, subset, nsub);
+  return zsum_blocks_per_pair(G, sg) * ((nav_of(G, subset, nsub) + 1) / 2);
 }
 
 namespace {
@@ -683,21 +705,22 @@ template <int SG>
 hipError_t zsum_launch(const DevGeom& G, const float* R, unsigned rbytes, const FwdArgs& o,
                        bool ratio, int storage, int subset, int nsub, const int* dsub, int a0,
                        int na, int nl, hipStream_t st) {
-  const unsigned zrb = (G.nrad + 127) / 128, zgb = (G.nsinogs + 4 * SG - 1) / (4 * SG);
-  const unsigned gz = zrb * zgb * na;
-  const unsigned off = zrb * zgb * a0;
+  // a0 is even (chunks hold whole pairs of angles)
+  const unsigned per = (unsigned)zsum_blocks_per_pair(G, SG);
+  const unsigned gz = per * (unsigned)((na + 1) / 2);
+  const unsigned off = per * (unsigned)(a0 / 2);
   if (!ratio) {
     hipLaunchKernelGGL((k_zsum<GFR_STORE_F32, false, false, SG>), dim3(gz), dim3(256), 0, st, G,
-                       R, rbytes, o, subset, nsub, dsub, a0, nl, off);
+                       R, rbytes, o, subset, nsub, dsub, a0, na, nl, off);
     return hipGetLastError();
   }
 #define GFR_ZS(S)                                                                             \
   if (o.llh_part)                                                                             \
     hipLaunchKernelGGL((k_zsum<S, true, true, SG>), dim3(gz), dim3(256), 0, st, G, R, rbytes, \
-                       o, subset, nsub, dsub, a0, nl, off);                                   \
+                       o, subset, nsub, dsub, a0, na, nl, off);                                   \
   else                                                                                        \
     hipLaunchKernelGGL((k_zsum<S, true, false, SG>), dim3(gz), dim3(256), 0, st, G, R,       \
-                       rbytes, o, subset, nsub, dsub, a0, nl, off);
+                       rbytes, o, subset, nsub, dsub, a0, na, nl, off);
   if (storage == GFR_STORE_F16) {
     GFR_ZS(GFR_STORE_F16)
   } else if (storage == GFR_STORE_BF16) {
@@ -711,10 +734,10 @@ hipError_t zsum_launch(const DevGeom& G, const float* R, unsigned rbytes, const 
 
 // Deterministic sum of the first n(subset) partials; with dsub the count
 // follows the device-side subset index.
-__global__ void k_llh_sum(const double* __restrict__ part, int per_angle, int nang, int subset,
+__global__ void k_llh_sum(const double* __restrict__ part, int per_pair, int nang, int subset,
                           int nsub, const int* __restrict__ dsub, double* __restrict__ out) {
   if (dsub) subset = *dsub;
-  const int n = per_angle * ((nang - subset + nsub - 1) / nsub);
+  const int n = per_pair * (((nang - subset + nsub - 1) / nsub + 1) / 2);
   __shared__ double red[256];
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) acc += part[i];
@@ -740,7 +763,10 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
   if (chunk < 1) return hipErrorInvalidValue;
   if (chunk > nav) chunk = nav;
   if ((size_t)chunk * per * sizeof(float) >= 0xFFFFFFF0u) chunk = (int)(0xFFFFFFF0u / (per * 4));
+  if (chunk < nav && chunk > 1) chunk &= ~1;  // the z-sum takes angles in pairs
   if (chunk < 1) return hipErrorInvalidValue;
+  // log-likelihood partials are indexed by pair: chunks must hold whole pairs
+  if (out.llh_part && chunk < nav && (chunk & 1)) return hipErrorInvalidValue;
   FwdArgs o{out.proj, out.corr, out.corr2, out.mask, out.y, out.bg, out.yscale, out.llh_part};
   const bool ratio = out.corr || out.corr2;
   const unsigned nrb = (G.nrad + kWave - 1) / kWave;
@@ -750,13 +776,13 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
     const int rz = tune.rot_z == 1 || tune.rot_z == 4 ? tune.rot_z : 16;
     const unsigned grot = nrb * ((nl + 3) / 4) * ((G.nzs + rz - 1) / rz) * na;
     if (rz == 1)
-      hipLaunchKernelGGL(k_rotate<1>, dim3(grot), dim3(kWave, 4), 0, st, G, img, R, subset, nsub,
+      hipLaunchKernelGGL(k_rotate<1>, dim3(grot), dim3(256), 0, st, G, img, R, subset, nsub,
                          dsub, a0, nl);
     else if (rz == 4)
-      hipLaunchKernelGGL(k_rotate<4>, dim3(grot), dim3(kWave, 4), 0, st, G, img, R, subset, nsub,
+      hipLaunchKernelGGL(k_rotate<4>, dim3(grot), dim3(256), 0, st, G, img, R, subset, nsub,
                          dsub, a0, nl);
     else
-      hipLaunchKernelGGL(k_rotate<16>, dim3(grot), dim3(kWave, 4), 0, st, G, img, R, subset, nsub,
+      hipLaunchKernelGGL(k_rotate<16>, dim3(grot), dim3(256), 0, st, G, img, R, subset, nsub,
                          dsub, a0, nl);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -772,9 +798,8 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
 
 hipError_t launch_llh_sum(const DevGeom& G, const double* part, int subset, int nsub,
                           const int* dsub, int sg, double* out, hipStream_t st) {
-  const int per_angle = ((G.nrad + 127) / 128) * ((G.nsinogs + 4 * sg - 1) / (4 * sg));
-  hipLaunchKernelGGL(k_llh_sum, dim3(1), dim3(256), 0, st, part, per_angle, G.nang, subset, nsub,
-                     dsub, out);
+  hipLaunchKernelGGL(k_llh_sum, dim3(1), dim3(256), 0, st, part, zsum_blocks_per_pair(G, sg),
+                     G.nang, subset, nsub, dsub, out);
   return hipGetLastError();
 }
 
@@ -847,7 +872,7 @@ hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* nume
 
 int llh_capacity(const GeomHost& h, int nsub) {
   const int nav = (h.g.nang + nsub - 1) / nsub;
-  return ((h.g.nrad + 127) / 128) * ((h.info.nsinogs + 15) / 16) * nav;  // zsum_sg = 4
+  return ((h.g.nrad + 127) / 128) * ((h.info.nsinogs + 15) / 16) * ((nav + 1) / 2);  // zsum_sg = 4
 }
 
 hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float* img, float* R,
@@ -905,7 +930,7 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
                            {1, 4, 0}, {1, 8, 0}, {1, 16, 0}};
   for (const auto& c : bcand) {
     Tune tt = t;
-    tt.back_lds = c[0] != 0;
+    tt.back_lds = c[0];
     tt.back_zb = c[1];
     tt.back_waves = c[2];
     const float ms = time_it([&] {

@@ -129,23 +129,23 @@ def test_back_project_c1_subset_vs_oracle(gctx, orc):
     close(n, ne, rtol=1e-6, atol=1e-5, what="numer c1")
     close(d, de, rtol=1e-6, atol=1e-5, what="denom c1")
     # the LDS-window backprojector on the same views (windows at full tilt)
-    gctx.set_tuning(g, 80, 8, 8, back_lds=True)
+    gctx.set_tuning(g, 80, 8, 8, back_lds=1)
     n2, d2 = gctx.back_project(g, corr, valid.astype(np.uint8), 5, 80)
     close(n2, ne, rtol=0, what="numer c1 lds")
     close(d2, de, rtol=0, what="denom c1 lds")
 
 
-@pytest.mark.parametrize("zb,sg,lds,bw,rz", [(4, 4, False, 0, 1), (8, 16, False, 0, 4),
-                                             (16, 8, False, 0, 16), (8, 8, False, 6, 4),
-                                             (4, 8, True, 0, 16), (8, 4, True, 0, 1),
-                                             (16, 16, True, 0, 4)])
+# back kind: 0 cached reads, 1 per-block LDS row window
+@pytest.mark.parametrize("zb,sg,lds,bw,rz", [(4, 4, 0, 0, 1), (8, 16, 0, 0, 4), (16, 8, 0, 0, 16),
+                                             (8, 8, 0, 6, 4), (4, 8, 1, 0, 16), (8, 4, 1, 0, 1),
+                                             (16, 16, 1, 0, 4)])
 def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg, lds, bw, rz):
     """Every autotuner candidate gives the oracle's bits (forward and back)."""
     g = C0
     gctx.set_tuning(g, 4, zb, sg, lds, bw, rz)
     t = gctx.tuning(g, 4)
     assert (t["back_zb"], t["zsum_sg"], t["back_lds"], t["back_waves"], t["rot_z"],
-            t["measured"]) == (zb, sg, int(lds), bw, rz, True)
+            t["measured"]) == (zb, sg, lds, bw, rz, True)
     x = rand_img(g, 30)
     close(gctx.forward_project(g, x, 2, 4), orc.forward(g, x, 2, 4), what="fwd sg=%d" % sg)
     rng = np.random.default_rng(31)
