@@ -121,7 +121,7 @@ class Context:
         d["measured"] = bool(rc)
         return d
 
-    def set_tuning(self, g, nsub, back_zb=8, zsum_sg=8, back_lds=False, back_waves=0, rot_z=4):
+    def set_tuning(self, g, nsub, back_zb=8, zsum_sg=8, back_lds=False, back_waves=0, rot_z=16):
         t = _lib.GfrTuning(back_zb, int(back_lds), back_waves, zsum_sg, rot_z, 0.0, 0.0)
         _check(self.h, lib().gfr_tuning_set(self.h, C.byref(g.c()), nsub, C.byref(t)))
 

@@ -560,8 +560,8 @@ int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, const gfr_tuni
     return set_err(ctx, GFR_EINVAL, "back_lds must be 0 or 1");
   if (in->back_waves != 0 && !(in->back_waves == 6 && in->back_zb == 8 && !in->back_lds))
     return set_err(ctx, GFR_EINVAL, "back_waves: 0, or 6 with 8 slices and direct reads");
-  if (in->rot_z != 1 && in->rot_z != 4 && in->rot_z != 16)
-    return set_err(ctx, GFR_EINVAL, "rot_z must be 1, 4 or 16");
+  if (in->rot_z != 4 && in->rot_z != 16 && in->rot_z != 32)
+    return set_err(ctx, GFR_EINVAL, "rot_z must be 4, 16 or 32");
   std::lock_guard<std::mutex> lk(ctx->mu);
   Tune& t = gd->tune[nsub];
   t.back_zb = in->back_zb;

@@ -50,7 +50,7 @@ struct Tune {
   int zsum_sg = 8;   // consecutive sinograms per z-sum wave: 4, 8 or 16
   int back_lds = 0;  // 1: backprojector reads its rows through an LDS window
   int back_waves = 0;     // register budget for this many waves per SIMD (0: compiler's)
-  int rot_z = 4;          // planes per rotation thread: 1, 4 or 16
+  int rot_z = 16;         // planes per rotation block: 4, 16 or 32
   bool tuned = false;
   float ms_back = 0.f, ms_zsum = 0.f;
 };

@@ -6,8 +6,8 @@
 // positions depend only on (theta, r, l); only z depends on the segment and
 // plane. Trilinear = lerp_z(bilinear_xy(plane z0), bilinear_xy(plane z1)), so
 // the kernel pair below computes every bilinear value once per angle:
-//   k_rotate: R[a][l][z][r] = bilinear_xy(image plane z, x(theta,r,l), y(theta,r,l))
-//   k_zsum  : P(s, r) = sum_l lerp(R[a][l][z0][r], R[a][l][z1][r], alpha_z)
+//   k_rotate_tile: R[a][l][z][r] = bilinear_xy(image plane z, x(theta,r,l), y(theta,r,l))
+//   k_zsum       : P(s, r) = sum_l lerp(R[a][l][z0][r], R[a][l][z1][r], alpha_z)
 // which is the same arithmetic, operation for operation, as sampling the
 // volume directly (bit-identical results). The z interpolation of sinogram s
 // at sample l, (z0, z1, alpha_z), is a per-geometry table, so a sample costs
@@ -41,57 +41,130 @@ __device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, unsigned voff,
 }
 
 // ---------------------------------------------------------------------------
-// k_rotate: one thread per (angle a of the chunk, sample l, bin r) and a run
-// of RZ planes (autotuned): the sample's x/y position and weights do not
-// depend on the plane, so they are computed once per run and the planes'
-// four texels are gathered at fixed offsets. Logical block order (r-block,
-// l-block, plane-run, a) with XCD-contiguous ranges.
+// k_rotate_tile: R[a][l][z][r], the bilinear sample of image plane z at the
+// position of (bin r, sample l) of angle a. A block takes one angle, a tile
+// of 32 bins x 32 samples and ZC planes (autotuned). The tile's sample
+// points fall in a rotated rectangle of the image; since x and y (and then
+// the clamped texel indices) are monotone in r and l, the texels the tile
+// touches lie in the bounding box of its four corner samples. Per plane
+// the block copies that box into LDS with coalesced row loads (the next
+// plane's rows are in flight while the current one is used) and every
+// output reads its four texels from LDS. A box larger than the LDS tile
+// (coarse geometries) falls back to gathers from global memory.
 // ---------------------------------------------------------------------------
-template <int RZ>
-__global__ __launch_bounds__(256) void k_rotate(DevGeom G, const float* __restrict__ img,
-                                                float* __restrict__ R, int subset, int nsub,
-                                                const int* __restrict__ dsub, int a0, int nl) {
-  constexpr int kRotZ = RZ;
+constexpr int kRotTile = 64;         // LDS box capacity: texels per row and rows
+constexpr int kRotStride = kRotTile + 1;  // LDS row stride (odd: rows on different banks)
+
+template <int ZC>
+__global__ __launch_bounds__(256) void k_rotate_tile(DevGeom G, const float* __restrict__ img,
+                                                     float* __restrict__ R, int subset, int nsub,
+                                                     const int* __restrict__ dsub, int a0,
+                                                     int nl) {
+  __shared__ float box[kRotTile * kRotStride];
   if (dsub) subset = *dsub;
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
-  const unsigned nrb = (G.nrad + kWave - 1) / kWave, nlb = (nl + 3) / 4;
-  const unsigned nzb = (G.nzs + kRotZ - 1) / kRotZ;
-  const unsigned rb = b % nrb;
-  unsigned q = b / nrb;
-  const unsigned lb = q % nlb;
-  q /= nlb;
-  const int zb = (int)(q % nzb) * kRotZ;
-  const int a = (int)(q / nzb);
-  // a wave covers 16 bins x 4 samples: its sample points form a compact
-  // patch of the image, so each gather touches few cache lines
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int r = rb * kWave + (tid >> 6) * 16 + (lane & 15);
-  const int li = lb * 4 + (lane >> 4);
-  if (r >= G.nrad || li >= nl || a0 + a >= (G.nang - subset + nsub - 1) / nsub) return;
+  const unsigned nrt = (G.nrad + 31) / 32, nlt = (nl + 31) / 32;
+  const unsigned nzc = (G.nzs + ZC - 1) / ZC;
+  const unsigned rt = b % nrt;
+  unsigned q = b / nrt;
+  const unsigned lt = q % nlt;
+  q /= nlt;
+  const int zc = (int)(q % nzc) * ZC;
+  const int a = (int)(q / nzc);
+  if (a0 + a >= (G.nang - subset + nsub - 1) / nsub) return;  // whole block
   const int it = subset + (a0 + a) * nsub;
   const float st = G.sin_t[it], ct = G.cos_t[it];
-  const float x0 = G.fxf * ((float)r - G.RADM);
-  const float xc = fmaf(x0, ct, G.RESM) + 0.5f;
-  const float yc = fmaf(x0, st, G.RESM) + 0.5f;
-  const float fl = (float)(G.l0 + li);
-  const float x = fmaf(fl, st, xc);   // dx = sin(theta)   (GFIRST_GPU_Kernels.cu:81)
-  const float y = fmaf(fl, -ct, yc);  // dy = -cos(theta)  (:82)
-  int x0i, x1i, y0i, y1i;
-  float ax, ay;
-  axis(x, G.res, x0i, x1i, ax);
-  axis(y, G.res, y0i, y1i, ay);
-  const int o00 = y0i * G.res + x0i, o01 = y0i * G.res + x1i;
-  const int o10 = y1i * G.res + x0i, o11 = y1i * G.res + x1i;
-  const size_t plane = (size_t)G.res * G.res;
-  const float* p = img + (size_t)zb * plane;
-  float* out = R + (((size_t)a * nl + li) * G.nzs + zb) * G.nrad + r;
-  const int nz = min(kRotZ, G.nzs - zb);
+  const int tid = threadIdx.x;
+  // texels and weights of sample (r, li)
+  auto texels = [&](int r, int li, int& x0i, int& x1i, float& ax, int& y0i, int& y1i,
+                    float& ay) {
+    const float x0 = G.fxf * ((float)r - G.RADM);
+    const float xc = fmaf(x0, ct, G.RESM) + 0.5f;
+    const float yc = fmaf(x0, st, G.RESM) + 0.5f;
+    const float fl = (float)(G.l0 + li);
+    axis(fmaf(fl, st, xc), G.res, x0i, x1i, ax);   // dx = sin(theta) (GFIRST_GPU_Kernels.cu:81)
+    axis(fmaf(fl, -ct, yc), G.res, y0i, y1i, ay);  // dy = -cos(theta) (:82)
+  };
+  const int rlo = (int)rt * 32, rhi = min(rlo + 31, G.nrad - 1);
+  const int llo = (int)lt * 32, lhi = min(llo + 31, nl - 1);
+  int bx0 = G.res, bx1 = -1, by0 = G.res, by1 = -1;
 #pragma unroll
-  for (int k = 0; k < kRotZ; k++) {
-    if (k < nz) {
-      const float* pk = p + (size_t)k * plane;
-      out[(size_t)k * G.nrad] =
-          lerp1(lerp1(pk[o00], pk[o01], ax), lerp1(pk[o10], pk[o11], ax), ay);
+  for (int c = 0; c < 4; c++) {
+    int x0i, x1i, y0i, y1i;
+    float ax, ay;
+    texels(c & 1 ? rhi : rlo, c & 2 ? lhi : llo, x0i, x1i, ax, y0i, y1i, ay);
+    bx0 = min(bx0, x0i);
+    bx1 = max(bx1, x1i);
+    by0 = min(by0, y0i);
+    by1 = max(by1, y1i);
+  }
+  const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
+  const bool staged = bw <= kRotTile && bh <= kRotTile;
+  // this thread's four outputs: bin rlo + (tid & 31), samples llo + (tid >> 5) + 8k
+  const int r = rlo + (tid & 31);
+  int o00[4], dx[4], dy[4];
+  float ax[4], ay[4];
+  bool ok[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int li = llo + (tid >> 5) + 8 * k;
+    ok[k] = r < G.nrad && li < nl;
+    int x0i, x1i, y0i, y1i;
+    texels(r, li, x0i, x1i, ax[k], y0i, y1i, ay[k]);
+    if (staged) {
+      o00[k] = (y0i - by0) * kRotStride + (x0i - bx0);
+      dx[k] = x1i - x0i;
+      dy[k] = (y1i - y0i) * kRotStride;
+    } else {
+      o00[k] = y0i * G.res + x0i;
+      dx[k] = x1i - x0i;
+      dy[k] = (y1i - y0i) * G.res;
+    }
+  }
+  const size_t plane = (size_t)G.res * G.res;
+  const int nz = min(ZC, G.nzs - zc);
+  float* out = R + (((size_t)a * nl + llo + (tid >> 5)) * G.nzs + zc) * G.nrad + r;
+  const size_t lstep = (size_t)8 * G.nzs * G.nrad;  // 8 samples further
+  if (!staged) {
+    for (int zi = 0; zi < nz; zi++) {
+      const float* pk = img + (size_t)(zc + zi) * plane;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (ok[k]) {
+          const float* p = pk + o00[k];
+          out[k * lstep + (size_t)zi * G.nrad] =
+              lerp1(lerp1(p[0], p[dx[k]], ax[k]), lerp1(p[dy[k]], p[dy[k] + dx[k]], ax[k]), ay[k]);
+        }
+      }
+    }
+    return;
+  }
+  // box loader: thread copies column tid & 63 of rows tid / 64 + 4 j
+  const int lc = tid & 63, lr = tid >> 6;
+  const int nj = (bh + 3) >> 2;
+  const float* src = img + (size_t)(by0 + lr) * G.res + bx0 + lc;
+  float pre[kRotTile / 4];
+  auto fetch = [&](int z) {
+    const float* pz = src + (size_t)z * plane;
+#pragma unroll
+    for (int j = 0; j < kRotTile / 4; j++)
+      if (j < nj && lc < bw && lr + 4 * j < bh) pre[j] = pz[(size_t)(4 * j) * G.res];
+  };
+  fetch(zc);
+  for (int zi = 0; zi < nz; zi++) {
+    __syncthreads();  // the previous plane's reads are done
+#pragma unroll
+    for (int j = 0; j < kRotTile / 4; j++)
+      if (j < nj && lc < bw && lr + 4 * j < bh) box[(lr + 4 * j) * kRotStride + lc] = pre[j];
+    __syncthreads();
+    if (zi + 1 < nz) fetch(zc + zi + 1);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (ok[k]) {
+        const float* p = box + o00[k];
+        out[k * lstep + (size_t)zi * G.nrad] =
+            lerp1(lerp1(p[0], p[dx[k]], ax[k]), lerp1(p[dy[k]], p[dy[k] + dx[k]], ax[k]), ay[k]);
+      }
     }
   }
 }
@@ -773,17 +846,16 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
   for (int a0 = 0; a0 < nav; a0 += chunk) {
     const int na = min(chunk, nav - a0);
     const unsigned rbytes = (unsigned)((size_t)na * per * sizeof(float));
-    const int rz = tune.rot_z == 1 || tune.rot_z == 4 ? tune.rot_z : 16;
-    const unsigned grot = nrb * ((nl + 3) / 4) * ((G.nzs + rz - 1) / rz) * na;
-    if (rz == 1)
-      hipLaunchKernelGGL(k_rotate<1>, dim3(grot), dim3(256), 0, st, G, img, R, subset, nsub,
-                         dsub, a0, nl);
-    else if (rz == 4)
-      hipLaunchKernelGGL(k_rotate<4>, dim3(grot), dim3(256), 0, st, G, img, R, subset, nsub,
-                         dsub, a0, nl);
+    const unsigned gxy = ((G.nrad + 31) / 32) * ((nl + 31) / 32) * na;
+    if (tune.rot_z == 4)
+      hipLaunchKernelGGL(k_rotate_tile<4>, dim3(gxy * ((G.nzs + 3) / 4)), dim3(256), 0, st, G, img,
+                         R, subset, nsub, dsub, a0, nl);
+    else if (tune.rot_z == 32)
+      hipLaunchKernelGGL(k_rotate_tile<32>, dim3(gxy * ((G.nzs + 31) / 32)), dim3(256), 0, st, G,
+                         img, R, subset, nsub, dsub, a0, nl);
     else
-      hipLaunchKernelGGL(k_rotate<16>, dim3(grot), dim3(256), 0, st, G, img, R, subset, nsub,
-                         dsub, a0, nl);
+      hipLaunchKernelGGL(k_rotate_tile<16>, dim3(gxy * ((G.nzs + 15) / 16)), dim3(256), 0, st, G,
+                         img, R, subset, nsub, dsub, a0, nl);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     switch (tune.zsum_sg) {
@@ -913,7 +985,7 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
       t.zsum_sg = c;
     }
   }
-  const int rcand[3] = {1, 4, 16};  // planes per rotation thread
+  const int rcand[3] = {4, 16, 32};  // planes per rotation block
   for (int c : rcand) {
     Tune tt = t;
     tt.rot_z = c;

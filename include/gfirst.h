@@ -133,7 +133,7 @@ typedef struct gfr_tuning {
   int back_lds;   /* 1: per-block LDS window of each view's rows; 0: cached reads */
   int back_waves; /* 6: 8-slice direct kernel built for 6 waves/SIMD; else 0     */
   int zsum_sg;    /* sinograms per forward z-sum wave: 4, 8 or 16                */
-  int rot_z;      /* planes per forward rotation thread: 1, 4 or 16             */
+  int rot_z;      /* planes per forward rotation block: 4, 16 or 32             */
   float ms_back;  /* measured time of the chosen backprojector (0 if not measured) */
   float ms_zsum;  /* measured time of the chosen forward projector              */
 } gfr_tuning;

@@ -72,6 +72,18 @@ def test_forward_c1_single_angle_vs_oracle(gctx, orc):
     close(gctx.forward_project(g, x, 37, 160), orc.forward(g, x, 37, 160), what="fwd c1 1 angle")
 
 
+def test_forward_wide_bins_vs_oracle(gctx, orc):
+    """Bins 4 voxels wide on a 96-voxel image: a rotation tile's texel box
+    exceeds the LDS tile at most angles (gathers from global memory), fits at
+    a few; every rotation shape gives the oracle's bits."""
+    g = SMALL.with_(res=96, nzs=8, nseg=3, fov=24.0)
+    x = rand_img(g, 12)
+    exp = orc.forward(g, x, 1, 3)
+    for rz in (4, 16, 32):
+        gctx.set_tuning(g, 3, rot_z=rz)
+        close(gctx.forward_project(g, x, 1, 3), exp, rtol=0, what="fwd wide bins rz=%d" % rz)
+
+
 def test_calib_mask_and_ratio(gctx, orc):
     g = C0
     rng = np.random.default_rng(4)
@@ -136,8 +148,8 @@ def test_back_project_c1_subset_vs_oracle(gctx, orc):
 
 
 # back kind: 0 cached reads, 1 per-block LDS row window
-@pytest.mark.parametrize("zb,sg,lds,bw,rz", [(4, 4, 0, 0, 1), (8, 16, 0, 0, 4), (16, 8, 0, 0, 16),
-                                             (8, 8, 0, 6, 4), (4, 8, 1, 0, 16), (8, 4, 1, 0, 1),
+@pytest.mark.parametrize("zb,sg,lds,bw,rz", [(4, 4, 0, 0, 32), (8, 16, 0, 0, 4), (16, 8, 0, 0, 16),
+                                             (8, 8, 0, 6, 4), (4, 8, 1, 0, 16), (8, 4, 1, 0, 32),
                                              (16, 16, 1, 0, 4)])
 def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg, lds, bw, rz):
     """Every autotuner candidate gives the oracle's bits (forward and back)."""
