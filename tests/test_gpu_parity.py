@@ -129,6 +129,27 @@ def test_back_project(gctx, orc, g):
         close(d, de, rtol=1e-6, atol=1e-6, what="denom")
 
 
+@pytest.mark.parametrize("g,s,S", [(C0, 1, 4), (C1, 3, 80)], ids=["c0", "c1"])
+def test_back_project_structured_mask(gctx, orc, g, s, S):
+    """Masks as a scanner makes them (valid inside the FOV, whole radial runs
+    masked at the edges) plus scattered dead bins: the oracle's bits. Ratios
+    include the -1000 gap value and negative counts."""
+    rng = np.random.default_rng(17)
+    nd = g.ndata
+    corr = rng.uniform(-0.2, 3, nd).astype(np.float32)
+    corr[rng.uniform(size=nd) < 0.05] = -1000.0
+    r = np.arange(g.nrad)
+    inside = np.abs(r - (g.nrad - 1) / 2) < 0.42 * g.nrad
+    valid = np.broadcast_to(inside, (g.nsinogs, g.nang, g.nrad)).copy()
+    dead = rng.integers(0, nd, size=max(8, nd // 20000))
+    valid.reshape(-1)[dead] = False
+    valid = valid.reshape(-1)
+    n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
+    ne, de = orc.back(g, corr, np.where(valid, 1, -1000).astype(np.float32), s, S)
+    close(n, ne, rtol=0, what="numer")
+    close(d, de, rtol=0, what="denom")
+
+
 def test_back_project_c1_subset_vs_oracle(gctx, orc):
     g = C1
     rng = np.random.default_rng(8)
