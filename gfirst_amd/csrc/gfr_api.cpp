@@ -554,8 +554,8 @@ int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, const gfr_tuni
   GFR_NONNULL(ctx, in, "tuning");
   if (nsub < 1 || nsub > g->nang) return set_err(ctx, GFR_EINVAL, "nsub out of range");
   auto ok = [](int v) { return v == 4 || v == 8 || v == 16; };
-  if (!ok(in->back_zb) || !ok(in->zsum_sg))
-    return set_err(ctx, GFR_EINVAL, "back_zb and zsum_sg must be 4, 8 or 16");
+  if (!(ok(in->back_zb) || in->back_zb == 5 || in->back_zb == 6) || !ok(in->zsum_sg))
+    return set_err(ctx, GFR_EINVAL, "back_zb must be 4, 5, 6, 8 or 16 and zsum_sg 4, 8 or 16");
   if (in->back_lds != 0 && in->back_lds != 1)
     return set_err(ctx, GFR_EINVAL, "back_lds must be 0 or 1");
   if (in->back_waves != 0 && !(in->back_waves == 6 && in->back_zb == 8 && !in->back_lds))

@@ -929,6 +929,8 @@ hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* nume
   }
   switch (tune.back_zb) {
     case 4: GFR_BK(4, 0); break;
+    case 5: GFR_BK(5, 0); break;
+    case 6: GFR_BK(6, 0); break;
     case 16: GFR_BK(16, 0); break;
     default:
       if (tune.back_waves == 6) {
@@ -998,8 +1000,8 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
     }
   }
   // backprojector: (kind, slices, register budget)
-  const int bcand[7][3] = {{0, 4, 0}, {0, 8, 0}, {0, 8, 6}, {0, 16, 0},
-                           {1, 4, 0}, {1, 8, 0}, {1, 16, 0}};
+  const int bcand[9][3] = {{0, 4, 0}, {0, 5, 0}, {0, 6, 0}, {0, 8, 0}, {0, 8, 6},
+                           {0, 16, 0}, {1, 4, 0}, {1, 8, 0}, {1, 16, 0}};
   for (const auto& c : bcand) {
     Tune tt = t;
     tt.back_lds = c[0];
