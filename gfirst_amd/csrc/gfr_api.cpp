@@ -553,9 +553,9 @@ int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, const gfr_tuni
   GFR_GEOM(ctx, g, gd);
   GFR_NONNULL(ctx, in, "tuning");
   if (nsub < 1 || nsub > g->nang) return set_err(ctx, GFR_EINVAL, "nsub out of range");
-  auto ok = [](int v) { return v == 4 || v == 8 || v == 16; };
-  if (!(ok(in->back_zb) || in->back_zb == 5 || in->back_zb == 6) || !ok(in->zsum_sg))
-    return set_err(ctx, GFR_EINVAL, "back_zb must be 4, 5, 6, 8 or 16 and zsum_sg 4, 8 or 16");
+  const int zb = in->back_zb, sg = in->zsum_sg;
+  if (!(zb == 4 || zb == 5 || zb == 6 || zb == 8 || zb == 16) || !(sg == 4 || sg == 6 || sg == 8))
+    return set_err(ctx, GFR_EINVAL, "back_zb must be 4, 5, 6, 8 or 16 and zsum_sg 4, 6 or 8");
   if (in->back_lds != 0 && in->back_lds != 1)
     return set_err(ctx, GFR_EINVAL, "back_lds must be 0 or 1");
   if (in->back_waves != 0 && !(in->back_waves == 6 && in->back_zb == 8 && !in->back_lds))

@@ -47,7 +47,7 @@ struct Taps9 { float g[9]; };
 // autotuner; none of them changes any summation order).
 struct Tune {
   int back_zb = 8;   // slices per backprojector thread: 4, 5, 6, 8 or 16
-  int zsum_sg = 8;   // consecutive sinograms per z-sum wave: 4, 8 or 16
+  int zsum_sg = 8;   // consecutive sinograms per z-sum wave: 4, 6 or 8
   int back_lds = 0;  // 1: backprojector reads its rows through an LDS window
   int back_waves = 0;     // register budget for this many waves per SIMD (0: compiler's)
   int rot_z = 16;         // planes per rotation block: 4, 16 or 32

@@ -860,7 +860,7 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
     if (e != hipSuccess) return e;
     switch (tune.zsum_sg) {
       case 4: e = zsum_launch<4>(G, R, rbytes, o, ratio, out.storage, subset, nsub, dsub, a0, na, nl, st); break;
-      case 16: e = zsum_launch<16>(G, R, rbytes, o, ratio, out.storage, subset, nsub, dsub, a0, na, nl, st); break;
+      case 6: e = zsum_launch<6>(G, R, rbytes, o, ratio, out.storage, subset, nsub, dsub, a0, na, nl, st); break;
       default: e = zsum_launch<8>(G, R, rbytes, o, ratio, out.storage, subset, nsub, dsub, a0, na, nl, st); break;
     }
     if (e != hipSuccess) return e;
@@ -974,7 +974,7 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
     return best;
   };
   Tune t;
-  const int cand[3] = {4, 8, 16};
+  const int cand[3] = {4, 6, 8};
   float bz = 1e30f, bb = 1e30f;
   for (int c : cand) {
     Tune tt = t;

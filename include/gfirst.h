@@ -132,7 +132,7 @@ typedef struct gfr_tuning {
   int back_zb;    /* slices per backprojector thread: 4, 5, 6, 8 or 16           */
   int back_lds;   /* 1: per-block LDS window of each view's rows; 0: cached reads */
   int back_waves; /* 6: 8-slice direct kernel built for 6 waves/SIMD; else 0     */
-  int zsum_sg;    /* sinograms per forward z-sum wave: 4, 8 or 16                */
+  int zsum_sg;    /* sinograms per forward z-sum wave: 4, 6 or 8                 */
   int rot_z;      /* planes per forward rotation block: 4, 16 or 32             */
   float ms_back;  /* measured time of the chosen backprojector (0 if not measured) */
   float ms_zsum;  /* measured time of the chosen forward projector              */
