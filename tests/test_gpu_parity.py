@@ -292,11 +292,15 @@ def make_study(g, seed, mode, counts=2e5):
     return d2, t2, norm, rec
 
 
-@pytest.mark.parametrize("g,mode,S,niter", [(C0, 1, 4, 2), (SMALL, 2, 3, 2), (SMALL, 3, 1, 2),
-                                             (SMALL, 3, 4, 1)],
-                         ids=["c0-m1-S4", "small-m2-S3", "small-m3-S1", "small-m3-S4"])
-def test_recon_vs_oracle(gctx, orc, g, mode, S, niter):
-    d2, t2, norm, rec = make_study(g, 12 + mode, mode)
+# c1-m1-S16: one whole iteration (16 subset passes) at the Inveon size, about
+# 15 s for the oracle on 16 host threads
+@pytest.mark.parametrize("g,mode,S,niter,counts", [(C0, 1, 4, 2, 2e5), (SMALL, 2, 3, 2, 2e5),
+                                                    (SMALL, 3, 1, 2, 2e5), (SMALL, 3, 4, 1, 2e5),
+                                                    (C1, 1, 16, 1, 5e7)],
+                         ids=["c0-m1-S4", "small-m2-S3", "small-m3-S1", "small-m3-S4",
+                              "c1-m1-S16"])
+def test_recon_vs_oracle(gctx, orc, g, mode, S, niter, counts):
+    d2, t2, norm, rec = make_study(g, 12 + mode, mode, counts)
     p = ga.recon_params(mode=mode, niter=niter, nsub=S)
     r = gctx.recon(g, p, d2, t2, norm, rec)
     pb = r.get(ga.OUT_PROJ_BLANK)
