@@ -175,6 +175,58 @@ __global__ __launch_bounds__(256) void k_jbf(int res, int nzs, const float* __re
   out[iV] = ((double)norm > 1e-8) ? sum / norm : 0.f;
 }
 
+// The reference's radius (4, GFIRST.cu:512) with the neighbour offsets as
+// loop counters: offsets and the spatial-weight index are the same for every
+// lane (scalar), a row's nine texels are loaded together from in-bounds
+// addresses, and each voxel still sums its neighbours in the reference's
+// order (z, then y, then x), skipping those past the volume's upper faces.
+__global__ __launch_bounds__(256) void k_jbf_r4(int res, int nzs, const float* __restrict__ in,
+                                                const float* __restrict__ guide,
+                                                const float* __restrict__ d_max, float max_ref,
+                                                double kI, GTab gt, float* __restrict__ out) {
+  constexpr int R = 4;
+  const int ix = blockIdx.x * kWave + threadIdx.x;
+  const int iy = blockIdx.y * 4 + threadIdx.y;
+  const int iz = blockIdx.z;
+  if (ix >= res || iy >= res) return;
+  const size_t iV = ((size_t)iz * res + iy) * res + ix;
+  if (ix < R || iy < R || iz < R) {
+    out[iV] = 0.f;
+    return;
+  }
+  const float mref = d_max ? __uint_as_float(*(const unsigned int*)d_max) : max_ref;
+  const float rv = guide[iV];
+  const long long r2 = (long long)res * res;
+  float norm = 0.f, sum = 0.f;
+  for (int dz = -R; dz <= R; dz++) {
+    for (int dy = -R; dy <= R; dy++) {
+      const bool vzy = iz + dz <= nzs - 1 && iy + dy <= res - 1;
+      const long long row = (long long)dz * r2 + (long long)dy * res;
+      float gv[2 * R + 1], iv[2 * R + 1];
+#pragma unroll
+      for (int k = 0; k <= 2 * R; k++) {
+        const bool v = vzy && ix + k - R <= res - 1;
+        const long long o = v ? row + (k - R) : 0;  // own voxel when outside
+        gv[k] = guide[iV + o];
+        iv[k] = in[iV + o];
+      }
+      const int d2yz = dz * dz + dy * dy;
+#pragma unroll
+      for (int k = 0; k <= 2 * R; k++) {
+        if (vzy && ix + k - R <= res - 1) {
+          const float tq = (gv[k] - rv) / mref;
+          const float di2 = tq * tq;
+          const float H = (float)exp(-(double)di2 / kI);
+          const float hg = H * gt.g[d2yz + (k - R) * (k - R)];
+          norm += hg;
+          sum = fmaf(hg, iv[k], sum);
+        }
+      }
+    }
+  }
+  out[iV] = ((double)norm > 1e-8) ? sum / norm : 0.f;
+}
+
 __global__ void k_blend(long long nv, float coef, const float* __restrict__ filt,
                         float* __restrict__ x) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -464,8 +516,12 @@ hipError_t launch_jbf(const DevGeom& G, const float* in, const float* guide,
   const double kS = (2.0 * (double)s3d) * (2.0 * (double)s3d);
   for (int d2 = 0; d2 < 193; d2++) gt.g[d2] = (float)exp(-(double)(float)d2 / kS);
   const double kI = (2.0 * (double)sI) * (2.0 * (double)sI);
-  hipLaunchKernelGGL(k_jbf, grid_img(G, G.nzs), dim3(kWave, 4), 0, st, G.res, G.nzs, in, guide,
-                     d_max_ref, max_ref, r, kI, gt, out);
+  if (r == 4)
+    hipLaunchKernelGGL(k_jbf_r4, grid_img(G, G.nzs), dim3(kWave, 4), 0, st, G.res, G.nzs, in,
+                       guide, d_max_ref, max_ref, kI, gt, out);
+  else
+    hipLaunchKernelGGL(k_jbf, grid_img(G, G.nzs), dim3(kWave, 4), 0, st, G.res, G.nzs, in,
+                       guide, d_max_ref, max_ref, r, kI, gt, out);
   return hipGetLastError();
 }
 
