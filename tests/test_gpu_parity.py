@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import gfirst_amd as ga
-from gfirst_amd.geometry import C0, C1, SMALL, TINY
+from gfirst_amd.geometry import C0, C1, SMALL, TINY, Geometry
 
 pytestmark = pytest.mark.gpu
 
@@ -82,6 +82,31 @@ def test_forward_wide_bins_vs_oracle(gctx, orc):
     for rz in (4, 16, 32):
         gctx.set_tuning(g, 3, rot_z=rz)
         close(gctx.forward_project(g, x, 1, 3), exp, rtol=0, what="fwd wide bins rz=%d" % rz)
+
+
+# image side not a multiple of the 16-column tiles, odd plane count, bins not
+# a multiple of the 32-bin rotation tiles: partial tiles in every kernel
+ODD = Geometry(40, 13, 30, 18, 3, 5, 2.0, 53.0, 40.0)
+
+
+def test_odd_geometry_operators(gctx, orc):
+    g = ODD
+    x = rand_img(g, 13)
+    close(gctx.forward_project(g, x), orc.forward(g, x), rtol=0, what="fwd odd S=1")
+    close(gctx.forward_project(g, x, 2, 5), orc.forward(g, x, 2, 5), rtol=0, what="fwd odd 2/5")
+    rng = np.random.default_rng(14)
+    nd = g.ndata
+    corr = rng.uniform(0, 3, nd).astype(np.float32)
+    corr[rng.uniform(size=nd) < 0.05] = -1000.0
+    valid = rng.uniform(size=nd) < 0.95
+    for s, S in ((0, 1), (4, 5)):
+        for zb in (4, 5, 6, 8, 16):
+            gctx.set_tuning(g, S, zb, 8)
+            n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
+            ne, de = orc.back(g, corr, np.where(valid, 1, -1000).astype(np.float32), s, S)
+            close(n, ne, rtol=0, what="numer odd zb=%d S=%d" % (zb, S))
+            close(d, de, rtol=0, what="denom odd zb=%d S=%d" % (zb, S))
+    close(gctx.blur_xy(g, x, 1.5), orc.blur_xy(g, x, 1.5), rtol=0, what="blur odd")
 
 
 def test_calib_mask_and_ratio(gctx, orc):
@@ -296,9 +321,9 @@ def make_study(g, seed, mode, counts=2e5):
 # 15 s for the oracle on 16 host threads
 @pytest.mark.parametrize("g,mode,S,niter,counts", [(C0, 1, 4, 2, 2e5), (SMALL, 2, 3, 2, 2e5),
                                                     (SMALL, 3, 1, 2, 2e5), (SMALL, 3, 4, 1, 2e5),
-                                                    (C1, 1, 16, 1, 5e7)],
+                                                    (ODD, 2, 5, 2, 1e5), (C1, 1, 16, 1, 5e7)],
                          ids=["c0-m1-S4", "small-m2-S3", "small-m3-S1", "small-m3-S4",
-                              "c1-m1-S16"])
+                              "odd-m2-S5", "c1-m1-S16"])
 def test_recon_vs_oracle(gctx, orc, g, mode, S, niter, counts):
     d2, t2, norm, rec = make_study(g, 12 + mode, mode, counts)
     p = ga.recon_params(mode=mode, niter=niter, nsub=S)
