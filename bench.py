@@ -97,6 +97,23 @@ def measured_traffic(dom, back_zb, storage, back_lds):
     return None if e is None else e["fetch_bytes"] + e["write_bytes"]
 
 
+def measured_pipes(dom, tune):
+    """Busy fractions of the CU's vector-memory units (texture addresser TA,
+    texture data TD) for the backprojector shape in use, from the committed
+    rocprofv3 counter runs; None when no measurement matches."""
+    path = os.path.join(ROOT, "profiles", "k_back_c1_pipes.json")
+    if dom != "back" or tune.get("back_lds") or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        m = json.load(f)
+    e = m.get("by_kernel", {}).get("k_back_direct<%d, %d, float>" % (tune["back_zb"],
+                                                                    tune["back_waves"]))
+    if e is None:
+        return None
+    return {"unit": "vector-memory path (TA address, TD data)", "ta_busy": e["ta_busy"],
+            "td_busy": e["td_busy"], "source": "profiles/k_back_c1_pipes.json"}
+
+
 def cpu_baseline(g, d2, norm, pb, nsub, nsteps):
     """The CPU oracle (FP32, OpenMP) on a bounded sample: nsteps
     subset-iterations of the same study (the untimed mask set-up reuses the
@@ -191,7 +208,8 @@ def run_c1(args):
                      "traffic": measured_traffic(dom, tune["back_zb"], storage, tune["back_lds"]),
                      "alg_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 4),
                      "fp32_tflops": round(flops, 2),
-                     "fp32_frac": round(flops / PEAK_FP32_TFLOPS, 4)},
+                     "fp32_frac": round(flops / PEAK_FP32_TFLOPS, 4),
+                     "binding": measured_pipes(dom, tune) if storage == 0 else None},
         "loglik_last_pass": ll,
         "tiles": tune,
     }
