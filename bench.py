@@ -7,6 +7,13 @@ U[0.8, 1.2]. A step is one subset-iteration: PSF blur, forward projection +
 ratio (+ FP64 log-likelihood), backprojection and image update of one
 subset, all on the device with the data resident in HBM.
 
+--workload c0: the small plain study (64x64x32 image, 64 bins x 64 angles
+x 32 rings, span 3, max ring difference 15; uniform 25 mm cylinder with three
+hot 4 mm spheres, 1e6 Poisson counts, normalization 1.0): the whole
+4-subset x 2-iteration reconstruction and the projection-only run timed on
+the GPU with the data resident; the oracle times the same reconstruction on
+the host and its image is compared with the GPU's bit for bit.
+
 --workload c3: the batched dynamic study. 64 frames of the same geometry,
 counts log-uniform in [1e5, 1e7], subset counts drawn from {8, 16, 24},
 activity = phantom channels weighted by a seeded two-compartment time curve,
@@ -137,6 +144,92 @@ def cpu_baseline(g, d2, norm, pb, nsub, nsteps):
                 sample="%d subset-iterations (S=%d) of the same Inveon-scale study, "
                        "oracle FP32 restatement with OpenMP" % (nsteps, nsub),
                 seconds=dt)
+
+
+def cpu_baseline_c0(g, d2, norm, nsub, niter, x_gpu):
+    """The oracle runs the same c0 reconstruction on the host (timed), and
+    its image is compared with the GPU's."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    orc.set_threads(threads)
+    t0 = time.perf_counter()
+    xD, _, _, _ = orc.osem(g, orc.params(mode=1, niter=niter, nsub=nsub), d2, None, norm)
+    dt = time.perf_counter() - t0
+    base = dict(value=niter * nsub / dt, unit="subset-iters/s", cores=orc.get_threads(),
+                kind="port", sample="the whole c0 reconstruction (%d subsets x %d iterations), "
+                "oracle FP32 restatement with OpenMP" % (nsub, niter), seconds=dt)
+    exact = int(np.sum(xD == x_gpu))
+    return base, {"bit_exact_voxels": exact, "voxels": int(xD.size)}
+
+
+def run_c0(args):
+    import gfirst_amd as ga
+    from gfirst_amd import synth
+    from gfirst_amd.geometry import C0
+    g = C0
+    nsub, niter = 4, 2
+    ctx = ga.Context(0)
+    img = synth.phantom_c0(g, seed=args.seed)
+    d_img = ctx.to_device(img)
+    d_blur = ctx.array(g.nvox)
+    ctx.blur_xy(g, d_img, 1.5, out=d_blur)
+    ybar = ctx.forward_project(g, d_blur)
+    y = synth.poisson_sinogram(ybar, 1e6, seed=args.seed + 1).astype(np.float32)
+    d2 = synth.as_file_pair(y)
+    norm = np.ones(g.ndata, np.float32)
+    # projection-only run (mode 1 of the reference): the phantom's sinogram
+    d_sino = ctx.array(g.ndata)
+    for _ in range(3):
+        ctx.forward_project(g, d_img, out=d_sino)
+    ctx.sync()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.forward_project(g, d_img, out=d_sino)
+    ctx.sync()
+    ms_proj = 1e3 * (time.perf_counter() - t0) / reps
+    # the reconstruction: session set-up (uploads, mask, autotuning) untimed
+    p = ga.recon_params(mode=1, niter=niter, nsub=nsub)
+    runs = []
+    for k in range(4):
+        r = ctx.recon(g, p, d2, None, norm)
+        r.sync()
+        t0 = time.perf_counter()
+        r.run()
+        r.sync()
+        runs.append(time.perf_counter() - t0)
+        if k == 3:
+            x = r.get(ga.OUT_XD)
+        r.close()
+    dt = min(runs[1:])
+    log("c0: reconstruction %.3f ms (%d subset-iterations), projection %.3f ms" % (
+        1e3 * dt, niter * nsub, ms_proj))
+    res = {
+        "metric": "c0 plain OSEM subset-iters/s",
+        "value": round(niter * nsub / dt, 2),
+        "unit": "subset-iters/s",
+        "n_gpus": 1,
+        "steps": niter * nsub,
+        "warmup": 1,
+        "ms_per_step": round(1e3 * dt / (niter * nsub), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded cylinder + 3 hot spheres, Poisson 1e6 counts, norm 1.0)",
+        "config": {"workload": "c0 plain OSEM: image 64x64x32, sinogram 64x64x%d, span 3, "
+                               "MRD 15, 4 subsets x 2 iterations" % g.nsinogs},
+        "reconstruction_ms": round(1e3 * dt, 3),
+        "projection_ms": round(ms_proj, 4),
+    }
+    if not args.no_cpu:
+        log("timing the oracle on the same reconstruction ...")
+        res["cpu_baseline"], res["parity"] = cpu_baseline_c0(g, d2, norm, nsub, niter, x)
+        log("cpu: %.2f subset-iters/s; %d/%d voxels bit-exact" % (
+            res["cpu_baseline"]["value"], res["parity"]["bit_exact_voxels"], res["parity"]["voxels"]))
+    ctx.close()
+    return res
 
 
 def run_c1(args):
@@ -607,7 +700,7 @@ def copy_probe(nbytes=1 << 30, reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c3", "c4"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c3", "c4"])
     ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--nsub", type=int, default=16)
@@ -633,6 +726,8 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         res = copy_probe()
+    elif args.workload == "c0":
+        res = run_c0(args)
     elif args.workload == "c3":
         res = run_c3(args)
     elif args.workload == "c4":
