@@ -14,6 +14,13 @@ hot 4 mm spheres, 1e6 Poisson counts, normalization 1.0): the whole
 the GPU with the data resident; the oracle times the same reconstruction on
 the host and its image is compared with the GPU's bit for bit.
 
+--workload c2: the joint doubles + triples study (mode 3) at the same
+Inveon size: doubles Poisson 5e7 counts from phantom channel 0, triples
+Poisson 1e6 counts from channel 1, 2e6 triple events binned to VLOR records;
+a step is one pass (one subset of one type: doubles, or triples with VLOR
+assignment, JBF and blend), timed over whole iterations. The oracle times one
+pass of each type.
+
 --workload c3: the batched dynamic study. 64 frames of the same geometry,
 counts log-uniform in [1e5, 1e7], subset counts drawn from {8, 16, 24},
 activity = phantom channels weighted by a seeded two-compartment time curve,
@@ -315,6 +322,168 @@ def run_c1(args):
                                                        res["cpu_baseline"]["cores"]))
     return res
 
+
+
+def c2_study(ctx, counts, nevents, seed):
+    """The joint study at Inveon size: doubles from phantom channel 0 and
+    triples from channel 1 (counts/50), both Poisson on the device from the
+    blurred phantoms' projections, and nevents triple events binned to VLOR
+    records on the host."""
+    from gfirst_amd import listmode, synth
+    from gfirst_amd.geometry import C1
+    g = C1
+
+    def expected(img):
+        d = ctx.to_device(img)
+        b = ctx.array(g.nvox)
+        ctx.blur_xy(g, d, 1.5, out=b)
+        p = ctx.array(g.ndata)
+        ctx.forward_project(g, b, out=p)
+        ctx.sync()
+        return p, float(p.download().sum(dtype=np.float64))
+
+    chA = synth.phantom_mouse(g, seed=seed, channel=0)
+    chB = synth.phantom_mouse(g, seed=seed, channel=1)
+    yD, sD = expected(chA)
+    yT, sT = expected(chB)
+    d2 = ctx.array(2 * g.ndata).zero()
+    ctx.poisson(yD, counts / sD, seed + 1, out=d2)
+    t2 = ctx.array(2 * g.ndata).zero()
+    ctx.poisson(yT, counts / 50.0 / sT, seed + 2, out=t2)
+    norm = synth.norm_uniform(g.ndata, seed=seed + 3)
+    hits = listmode.triple_events(g, chB, nevents, seed=seed + 5)
+    rec = listmode.bin_hits(g, hits, 80.5)
+    rec = rec[(rec[:, 0] > 0) & (rec[:, 1] > 0)]
+    ctx.sync()
+    return g, d2, t2, norm, rec
+
+
+def cpu_baseline_c2(g, d2, t2, norm, rec, pb, nsub):
+    """The oracle on a bounded sample of the joint study: one doubles pass
+    and one triples pass (subset 0 of the first iteration, as in the
+    reference's loop GFIRST.cu:324-527: blur, forward, VLOR assignment for
+    the triples, ratio, back, update, then JBF and blend). The set-up (mask
+    from the device's PROJ_BLANK, VLOR precount and blank, the triples
+    projection cache) is untimed."""
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    orc.set_threads(threads)
+    nd, nv = g.ndata, g.nvox
+    res = np.float32(g.res)
+    cal = orc.calib(g, pb, norm)
+    yD, bD = res * d2[:nd], res * d2[nd:]
+    yT, bT = res * t2[:nd], res * t2[nd:]
+    v0 = orc.vlor_precount(g, rec)
+    blankt = orc.vlor_blankt(g, yT, v0)
+    xD = orc.cylinder(g, 1.0)
+    xT = orc.cylinder(g, 1.0)
+    ptc = orc.forward(g, orc.blur_xy(g, xT, 1.5), 0, 1)
+    dv = np.zeros(nd, np.float32)
+    sub = np.zeros((g.nsinogs, g.nang), bool)
+    sub[:, 0::nsub] = True
+    sub = np.repeat(sub.reshape(-1), g.nrad)
+    t0 = time.perf_counter()
+    xb = orc.blur_xy(g, xD, 1.5)
+    proj = orc.forward(g, xb, 0, nsub)
+    corr = orc.ratio(g, proj, yD, bD, 0, nsub)
+    num, den = orc.back(g, corr, cal, 0, nsub)
+    xD = orc.update(g, xD, num, den, 1.5)
+    t1 = time.perf_counter()
+    xb = orc.blur_xy(g, xT, 1.5)
+    proj = orc.forward(g, xb, 0, nsub)
+    ptc[sub] = proj[sub]
+    dv = orc.vlor_assign(g, rec, blankt, ptc, 0, nsub, dv)
+    corr = orc.ratio(g, proj, dv * res, bT, 0, nsub)
+    num, den = orc.back(g, corr, cal, 0, nsub)
+    xT = orc.update(g, xT, num, den, 1.5)
+    filt = orc.jbf(g, xT, xD, orc.vmax(xD))
+    xT = orc.blend(g, xT, filt)
+    t2_ = time.perf_counter()
+    dt = t2_ - t0
+    return dict(value=2 / dt, unit="subset-iters/s", cores=orc.get_threads(), kind="port",
+                sample="one doubles pass (%.1f s) and one triples pass with VLOR assignment, "
+                       "JBF and blend (%.1f s), subset 0 of %d of the same joint study, oracle "
+                       "FP32 restatement with OpenMP" % (t1 - t0, t2_ - t1, nsub),
+                seconds=dt)
+
+
+def run_c2(args):
+    import gfirst_amd as ga
+    storage = STORAGE[args.storage or "f32"]
+    nsub = args.nsub
+    ctx = ga.Context(0)
+    t = time.perf_counter()
+    g, d2, t2, norm, rec = c2_study(ctx, args.counts, args.events, args.seed)
+    log("c2: joint study ready in %.1fs (%d VLOR records)" % (time.perf_counter() - t, len(rec)))
+    per_iter = 2 * nsub  # a doubles and a triples pass per subset
+    warm = -(-args.warmup // per_iter) * per_iter
+    steps = -(-args.steps // per_iter) * per_iter
+    niter = (warm + steps) // per_iter
+    p = ga.recon_params(mode=3, niter=niter, nsub=nsub, storage=storage)
+    r = ctx.recon(g, p, d2, t2, norm, rec)
+    tune = ctx.tuning(g, nsub)
+    log("c2: %d iterations, %d passes each; resident %.2f GB" % (
+        niter, per_iter, ctx.bytes_in_use() / 1e9))
+    for _ in range(warm):
+        r.step()
+    r.sync()
+    r.profile(True)
+    r.phase_times()
+    r.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.step()
+    r.sync()
+    dt = time.perf_counter() - t0
+    ph = r.phase_times()
+    r.profile(False)
+    cur = r.cursor()
+    ms_step = 1e3 * dt / steps
+    log("c2: timed %d passes: %.3f ms/pass; phases %s" % (
+        steps, ms_step, {k: round(v[0], 2) for k, v in ph.items()}))
+    sb = 4 if storage == ga.STORE_F32 else 2
+    ab = alg_bytes(g, nsub, sb, has_bg=False)
+    dom = max(("forward", "back"), key=lambda k: ph[k][0])
+    avg_ms = ph[dom][0] / max(ph[dom][1], 1)
+    achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
+    res = {
+        "metric": "joint doubles+triples OSEM subset-iters/s",
+        "value": round(steps / dt, 3),
+        "unit": "subset-iters/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if storage == ga.STORE_F32 else args.storage + "-storage/f32",
+        "data": "synthetic (seeded mouse phantom channels, device Poisson %.3g doubles / %.3g "
+                "triples counts, %d triple events binned to %d VLOR records, norm U[0.8,1.2])" % (
+                    args.counts, args.counts / 50.0, args.events, len(rec)),
+        "config": {"workload": "c2 joint doubles+triples OSEM (mode 3) with VLOR assignment: "
+                               "image 128x128x159, sinogram 128x160x4319, span 3, MRD 79, "
+                               "%d subsets; a step is one pass (one subset of one type)" % nsub,
+                   "subsets": nsub, "storage": args.storage or "f32"},
+        "phase_ms_per_iteration": {k: round(v[0] * per_iter / steps, 3) for k, v in ph.items()
+                                   if v[1]},
+        "roofline": {"kernel": "k_forward" if dom == "forward" else "k_back", "bound": "hbm",
+                     "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
+                     "alg_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 4)},
+        "cursor_after": cur,
+        "tiles": tune,
+    }
+    pb = r.get(ga.OUT_PROJ_BLANK)
+    r.close()
+    if not args.no_cpu:
+        log("timing the oracle on one doubles and one triples pass ...")
+        res["cpu_baseline"] = cpu_baseline_c2(g, d2.download(), t2.download(), norm, rec, pb,
+                                              nsub)
+        log("cpu: %.3f subset-iters/s (%s)" % (res["cpu_baseline"]["value"],
+                                              res["cpu_baseline"]["sample"]))
+    ctx.close()
+    return res
 
 
 def c3_inputs(ctx, g, seed):
@@ -700,7 +869,7 @@ def copy_probe(nbytes=1 << 30, reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c3", "c4"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--nsub", type=int, default=16)
@@ -719,6 +888,7 @@ def main():
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--events", type=int, default=2000000, help="c2: triple events")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -728,6 +898,8 @@ def main():
         res = copy_probe()
     elif args.workload == "c0":
         res = run_c0(args)
+    elif args.workload == "c2":
+        res = run_c2(args)
     elif args.workload == "c3":
         res = run_c3(args)
     elif args.workload == "c4":
