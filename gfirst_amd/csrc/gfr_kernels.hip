@@ -177,47 +177,120 @@ __global__ __launch_bounds__(256) void k_jbf(int res, int nzs, const float* __re
 
 // The reference's radius (4, GFIRST.cu:512) with the neighbour offsets as
 // loop counters: offsets and the spatial-weight index are the same for every
-// lane (scalar), a row's nine texels are loaded together from in-bounds
-// addresses, and each voxel still sums its neighbours in the reference's
-// order (z, then y, then x), skipping those past the volume's upper faces.
+// lane (scalar), a row's nine texels are read by buffer loads from one 32-bit
+// row offset (texels past the volume's end read 0 and are not summed), and
+// each voxel still sums its neighbours in the reference's order (z, then y,
+// then x), skipping those past the volume's upper faces.
+//
+// Each intensity weight H = (float)exp(-(double)di2 / kI) is first computed
+// on a fast path whose result is provably the same float:
+//  - tq = (g - rv) / mref by a reciprocal and two FMA corrections with
+//    y = RN(1/mref) (Markstein's theorem: the second correction of a
+//    faithful quotient is the correctly rounded one). It holds while nothing
+//    underflows or overflows: for mref in [2^-20, 2^20) and a first quotient
+//    q0 of magnitude in [2^-60, 2^90). Outside that range the weight is the
+//    same on both paths when kI is in [2^-80, 2^40]: |q0| < 2^-60 gives
+//    |x| < 2^-40 and H = 1; |q0| >= 2^90 gives di2 = inf (H = 0) or a NaN,
+//    which the range test below sends to the reference expression. With
+//    mref or kI outside those bounds every weight takes that expression.
+//  - x' = di2 * (-1/kI) instead of the division (|x' - x| <= 3 ulp(x)), and
+//    exp(x') = 2^(n/1024) * p(r): n rounded by the 1.5 * 2^52 shift (its low
+//    word is n), r = x' - n ln2/1024 in one FMA, a 1024-entry table of
+//    2^(j/1024) and a cubic on |r| <= ln2/2048. For x' >= -87 (a normal
+//    float result) the relative error is below 2^-44.
+//  - the float rounding of E drops its 29 low mantissa bits; if they are
+//    more than 2^13 units (2^-40 relative) away from the halfway point, the
+//    library's exp of the exact x rounds to the same float. Below x' = -104.5
+//    the float is 0 either way.
+// A lane that fails any test (about 3e-5 of them) recomputes the weight by
+// the reference's expression; the row's weights are then summed in order.
+namespace jbfx {
+constexpr int kBits = 10, kTab = 1 << kBits;
+constexpr double kCN = 1477.3197218702985;                  // 1024 / ln2
+constexpr double kShift = 0x1.8p52;                          // ulp 1
+constexpr double kLn2N = 0x1.62e42fefa39efp-1 / kTab;
+constexpr unsigned kHalf = 1u << 28, kBand = 1u << 13;
+}  // namespace jbfx
+
 __global__ __launch_bounds__(256) void k_jbf_r4(int res, int nzs, const float* __restrict__ in,
                                                 const float* __restrict__ guide,
                                                 const float* __restrict__ d_max, float max_ref,
-                                                double kI, GTab gt, float* __restrict__ out) {
+                                                double kI, double nki, GTab gt,
+                                                float* __restrict__ out) {
   constexpr int R = 4;
+  __shared__ double p2[jbfx::kTab];  // 2^(j/1024)
+  for (int j = threadIdx.y * kWave + threadIdx.x; j < jbfx::kTab; j += 4 * kWave)
+    p2[j] = exp2((double)j * (1.0 / jbfx::kTab));
+  __syncthreads();
   const int ix = blockIdx.x * kWave + threadIdx.x;
   const int iy = blockIdx.y * 4 + threadIdx.y;
   const int iz = blockIdx.z;
   if (ix >= res || iy >= res) return;
-  const size_t iV = ((size_t)iz * res + iy) * res + ix;
+  const int r2 = res * res;
+  const int iV = (iz * res + iy) * res + ix;
   if (ix < R || iy < R || iz < R) {
     out[iV] = 0.f;
     return;
   }
+  const unsigned vbytes = (unsigned)(r2 * nzs) * 4u;
+  const __amdgpu_buffer_rsrc_t rg = rsrc(guide, vbytes), ri = rsrc(in, vbytes);
   const float mref = d_max ? __uint_as_float(*(const unsigned int*)d_max) : max_ref;
+  const float ym = 1.0f / mref;
+  const unsigned allslow =
+      (mref >= 0x1p-20f && mref < 0x1p20f && kI >= 0x1p-80 && kI <= 0x1p40) ? 0u : 0x1ffu;
   const float rv = guide[iV];
-  const long long r2 = (long long)res * res;
   float norm = 0.f, sum = 0.f;
   for (int dz = -R; dz <= R; dz++) {
     for (int dy = -R; dy <= R; dy++) {
       const bool vzy = iz + dz <= nzs - 1 && iy + dy <= res - 1;
-      const long long row = (long long)dz * r2 + (long long)dy * res;
-      float gv[2 * R + 1], iv[2 * R + 1];
+      // bit k: neighbour ix + k - R lies inside the volume
+      const unsigned vmask = vzy ? (0x1ffu >> max(0, ix + R - (res - 1))) : 0u;
+      unsigned voff = (unsigned)(iV + dz * r2 + dy * res - R) * 4u;
+      asm volatile("" : "+v"(voff));  // row start as the base: texel k at immediate 4k
+      float gv[2 * R + 1], iv[2 * R + 1], H[2 * R + 1];
 #pragma unroll
       for (int k = 0; k <= 2 * R; k++) {
-        const bool v = vzy && ix + k - R <= res - 1;
-        const long long o = v ? row + (k - R) : 0;  // own voxel when outside
-        gv[k] = guide[iV + o];
-        iv[k] = in[iV + o];
+        gv[k] = ld_f1(rg, voff + 4u * k, 0);
+        iv[k] = ld_f1(ri, voff + 4u * k, 0);
+      }
+      unsigned slow = 0;
+#pragma unroll
+      for (int k = 0; k <= 2 * R; k++) {
+        const float a = gv[k] - rv;
+        const float q0 = a * ym;
+        const float q1 = fmaf(fmaf(-q0, mref, a), ym, q0);
+        const float tq = fmaf(fmaf(-q1, mref, a), ym, q1);
+        const float di2 = tq * tq;
+        const double x = (double)di2 * nki;
+        const double sh = fma(x, jbfx::kCN, jbfx::kShift);
+        const int n = __double2loint(sh);
+        const double r = fma(-(sh - jbfx::kShift), jbfx::kLn2N, x);
+        double p = fma(r, 1.0 / 6, 0.5);
+        p = fma(p, r, 1.0);
+        p = fma(p, r, 1.0);
+        const double E = ldexp(p2[n & (jbfx::kTab - 1)] * p, n >> jbfx::kBits);
+        const unsigned low = (unsigned)__double2loint(E) & 0x1fffffffu;
+        const bool amb = low - (jbfx::kHalf - jbfx::kBand) < 2u * jbfx::kBand;
+        const bool under = x < -104.5;
+        H[k] = under ? 0.f : (float)E;
+        const bool bad = !under & (!(x >= -87.0) | amb);
+        slow |= bad ? 1u << k : 0u;
+      }
+      slow = (slow | allslow) & vmask;
+      if (slow) {
+        for (int k = 0; k <= 2 * R; k++) {
+          if ((slow >> k) & 1u) {
+            const float tq = (gv[k] - rv) / mref;
+            const float di2 = tq * tq;
+            H[k] = (float)exp(-(double)di2 / kI);
+          }
+        }
       }
       const int d2yz = dz * dz + dy * dy;
 #pragma unroll
       for (int k = 0; k <= 2 * R; k++) {
-        if (vzy && ix + k - R <= res - 1) {
-          const float tq = (gv[k] - rv) / mref;
-          const float di2 = tq * tq;
-          const float H = (float)exp(-(double)di2 / kI);
-          const float hg = H * gt.g[d2yz + (k - R) * (k - R)];
+        if ((vmask >> k) & 1u) {
+          const float hg = H[k] * gt.g[d2yz + (k - R) * (k - R)];
           norm += hg;
           sum = fmaf(hg, iv[k], sum);
         }
@@ -516,9 +589,10 @@ hipError_t launch_jbf(const DevGeom& G, const float* in, const float* guide,
   const double kS = (2.0 * (double)s3d) * (2.0 * (double)s3d);
   for (int d2 = 0; d2 < 193; d2++) gt.g[d2] = (float)exp(-(double)(float)d2 / kS);
   const double kI = (2.0 * (double)sI) * (2.0 * (double)sI);
-  if (r == 4)
+  // the radius-4 kernel addresses the volume with 32-bit byte offsets
+  if (r == 4 && G.nvox < (1ll << 29))
     hipLaunchKernelGGL(k_jbf_r4, grid_img(G, G.nzs), dim3(kWave, 4), 0, st, G.res, G.nzs, in,
-                       guide, d_max_ref, max_ref, kI, gt, out);
+                       guide, d_max_ref, max_ref, kI, -1.0 / kI, gt, out);
   else
     hipLaunchKernelGGL(k_jbf, grid_img(G, G.nzs), dim3(kWave, 4), 0, st, G.res, G.nzs, in,
                        guide, d_max_ref, max_ref, r, kI, gt, out);

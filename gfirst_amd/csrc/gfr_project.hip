@@ -27,19 +27,6 @@ namespace gfr {
 
 namespace {
 
-// Raw buffer resource over [p, p + bytes): 32-bit offsets, loads past the end
-// return 0 without touching memory.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, unsigned bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
-                                           0x00020000);
-}
-__device__ __forceinline__ float2 ld_f2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-__device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
-}
-
 // ---------------------------------------------------------------------------
 // k_rotate_tile: R[a][l][z][r], the bilinear sample of image plane z at the
 // position of (bin r, sample l) of angle a. A block takes one angle, a tile

@@ -280,6 +280,45 @@ def test_jbf_blend_max(gctx, orc):
     close(b, orc.blend(g, inp, exp, 0.1), rtol=0, what="blend")
 
 
+def near_midpoint_t(n, kI, seed=0):
+    """Guide differences t whose intensity weight exp(-t*t/kI) lies within
+    2^-42 (and beyond 2^-50) relative of a midpoint between two floats:
+    the weights whose float rounding a less accurate exp could flip."""
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(0.001, 0.45, n).astype(np.float32)
+    E = np.exp(-(t * t).astype(np.float64) / kI)
+    H = E.astype(np.float32)
+    Hd = H.astype(np.float64)
+    up = np.nextafter(H, np.float32(np.inf)).astype(np.float64)
+    dn = np.nextafter(H, np.float32(0)).astype(np.float64)
+    d = np.minimum(np.abs(E - (Hd + up) / 2), np.abs(E - (Hd + dn) / 2)) / E
+    return t[(d > 2.0 ** -50) & (d < 2.0 ** -42) & (H > 0)]
+
+
+@pytest.mark.parametrize("scale", [1.0, 2.0 ** 24], ids=["mref1", "mref2^24"])
+def test_jbf_weights_near_rounding_midpoints(gctx, orc, scale):
+    """Radius-4 JBF on guides whose weights sit next to float rounding
+    midpoints (and a few denormal guide values), against the oracle bit for
+    bit. With max_ref = 2^24 every weight takes the reference expression."""
+    g = SMALL
+    t = near_midpoint_t(8_000_000, (2.0 * 0.025) ** 2)
+    assert len(t) >= 30
+    guide = np.zeros((g.nzs, g.res, g.res), np.float32)
+    slots = [(z, y, x) for z in (5, 10, 15) for y in range(5, 28, 5) for x in range(5, 28, 5)]
+    for (z, y, x), v in zip(slots, t):
+        guide[z, y, x] = v
+    guide[12, 6, 7] = np.float32(1e-42)   # denormal
+    guide[8, 22, 21] = np.float32(3e-39)
+    guide = (guide * np.float32(scale)).reshape(-1)
+    rng = np.random.default_rng(12)
+    inp = rng.uniform(0.5, 2, g.nvox).astype(np.float32)
+    mref = float(np.float32(scale))
+    got = gctx.jbf3d(g, inp, guide, mref)
+    exp = orc.jbf(g, inp, guide, mref)
+    close(got, exp, rtol=0, atol=0, what="jbf near midpoints")
+    assert np.array_equal(got, exp)
+
+
 def test_vlor_assign(gctx, orc):
     g = SMALL
     nd = g.ndata
