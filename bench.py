@@ -822,6 +822,7 @@ def run_c4(args):
         xD, _ = srv.result(rid)
         same[name] = bool(np.array_equal(r.get(ga.OUT_XD), xD))
         r.close()
+    err = c4_error_report(ctx, srv, cls, storage, args.seed) if args.error_report else None
     srv.close()
     res = {
         "metric": "p50/p99 recon latency under load",
@@ -851,8 +852,43 @@ def run_c4(args):
         "drain_s": round(t_end - t0 - args.duration, 2),
         "alone_equals_mix": same,
     }
+    if err is not None:
+        res["error_vs_fp64_oracle"] = err
     ctx.close()
     return res
+
+
+def c4_error_report(ctx, srv, cls, storage, seed):
+    """Per request class, one served request (1 iteration, 1e7 counts, the
+    stream's storage format) against the FP64 oracle's reconstruction of the
+    same data: relative L2 image error of the doubles (and triples) image."""
+    import gfirst_amd as ga
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    orc.set_threads(threads)
+    out = {}
+    for k, name in enumerate(cls):
+        c = cls[name]
+        d2, t2 = c4_inputs(ctx, c, 1e7, 10 ** 7 + k)
+        p = ga.recon_params(mode=c["mode"], niter=1, nsub=c["nsub"], storage=storage)
+        rid = srv.submit(c["g"], p, d2, t2, c["norm"], c.get("rec"))
+        srv.wait(rid)
+        xD, xT = srv.result(rid)
+        srv.release(rid)
+        log("c4 error report: %s served; FP64 oracle (mode %d, S=%d, 1 iteration) ..." % (
+            name, c["mode"], c["nsub"]))
+        t0 = time.perf_counter()
+        oD, oT, _, _ = orc.osem(c["g"], orc.params(mode=c["mode"], niter=1, nsub=c["nsub"]),
+                                d2.download(), None if t2 is None else t2.download(),
+                                c["norm"].download(), c.get("rec"), fp64=True)
+        e = {"mode": c["mode"], "subsets": c["nsub"], "iterations": 1, "counts": 1e7,
+             "doubles": float(np.linalg.norm(xD - oD) / np.linalg.norm(oD)),
+             "oracle_s": round(time.perf_counter() - t0, 1)}
+        if xT is not None:
+            e["triples"] = float(np.linalg.norm(xT - oT) / np.linalg.norm(oT))
+        out[name] = e
+        log("c4 error report: %s %s" % (name, e))
+    return out
 
 
 def copy_probe(nbytes=1 << 30, reps=5):
