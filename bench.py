@@ -861,7 +861,8 @@ def run_c4(args):
 def c4_error_report(ctx, srv, cls, storage, seed):
     """Per request class, one served request (1 iteration, 1e7 counts, the
     stream's storage format) against the FP64 oracle's reconstruction of the
-    same data: relative L2 image error of the doubles (and triples) image."""
+    same data: relative L2 image error of the doubles (and triples) image,
+    beside the FP32 oracle's own distance from the FP64 oracle."""
     import gfirst_amd as ga
     from oracle import oracle as orc
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
@@ -886,6 +887,13 @@ def c4_error_report(ctx, srv, cls, storage, seed):
              "oracle_s": round(time.perf_counter() - t0, 1)}
         if xT is not None:
             e["triples"] = float(np.linalg.norm(xT - oT) / np.linalg.norm(oT))
+        # the reference's own FP32 arithmetic (the FP32 oracle) on the same data
+        fD, fT, _, _ = orc.osem(c["g"], orc.params(mode=c["mode"], niter=1, nsub=c["nsub"]),
+                                d2.download(), None if t2 is None else t2.download(),
+                                c["norm"].download(), c.get("rec"))
+        e["fp32_oracle_doubles"] = float(np.linalg.norm(fD - oD) / np.linalg.norm(oD))
+        if xT is not None:
+            e["fp32_oracle_triples"] = float(np.linalg.norm(fT - oT) / np.linalg.norm(oT))
         out[name] = e
         log("c4 error report: %s %s" % (name, e))
     return out

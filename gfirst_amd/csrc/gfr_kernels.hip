@@ -202,8 +202,9 @@ __global__ __launch_bounds__(256) void k_jbf(int res, int nzs, const float* __re
 //    more than 2^13 units (2^-40 relative) away from the halfway point, the
 //    library's exp of the exact x rounds to the same float. Below x' = -104.5
 //    the float is 0 either way.
-// A lane that fails any test (about 3e-5 of them) recomputes the weight by
-// the reference's expression; the row's weights are then summed in order.
+// A weight near a rounding midpoint (about 3e-5 of them) or with x' in
+// [-104.5, -87) (a denormal float) is recomputed by the reference's
+// expression; the row's weights are then summed in order.
 namespace jbfx {
 constexpr int kBits = 10, kTab = 1 << kBits;
 constexpr double kCN = 1477.3197218702985;                  // 1024 / ln2
