@@ -415,3 +415,60 @@ def test_errors_are_status_codes(gctx):
         small.recon(C0, ga.recon_params(), d2, None, norm)
     assert e.value.code == ga.GFR_ENOMEM
     small.close()
+
+
+def test_joint_inveon_first_triples_passes_vs_oracle(gctx, orc):
+    """The joint reconstruction (mode 3) at the Inveon size, 860k VLOR records
+    from 2e6 triple events: the session's first 18 passes (16 doubles
+    subsets, then triples subsets 0 and 1 with VLOR assignment, projection
+    cache, bilateral filter and blend) against the oracle following the
+    reference's loop (GFIRST.cu:324-527) pass by pass, bit for bit.
+    About 30 s of oracle work on 16 host threads."""
+    from gfirst_amd import listmode, synth
+    g, S = C1, 16
+    nd = g.ndata
+
+    def expected(img):
+        b = gctx.blur_xy(g, img, 1.5)
+        return gctx.forward_project(g, b).astype(np.float64)
+
+    chA = synth.phantom_mouse(g, seed=3, channel=0)
+    chB = synth.phantom_mouse(g, seed=3, channel=1)
+    yDe, yTe = expected(chA), expected(chB)
+    rng = np.random.default_rng(31)
+    d2 = synth.as_file_pair(rng.poisson(yDe * (1e7 / yDe.sum())).astype(np.float32))
+    t2 = synth.as_file_pair(rng.poisson(yTe * (2e5 / yTe.sum())).astype(np.float32))
+    norm = synth.norm_uniform(nd, seed=5)
+    rec = listmode.bin_hits(g, listmode.triple_events(g, chB, 2000000, seed=8), 80.5)
+    rec = rec[(rec[:, 0] > 0) & (rec[:, 1] > 0)]
+    r = gctx.recon(g, ga.recon_params(mode=3, niter=1, nsub=S), d2, t2, norm, rec)
+    for _ in range(S + 2):
+        r.step()
+    c = r.cursor()
+    assert (c["type"], c["subset"], c["passes"]) == (1, 2, S + 2)
+    xD, xT = r.get(ga.OUT_XD), r.get(ga.OUT_XT)
+    r.close()
+
+    res = np.float32(g.res)
+    cal = orc.calib(g, orc.forward(g, orc.cylinder(g, 1.0), 0, 1), norm)
+    yD, bD = res * d2[:nd], res * d2[nd:]
+    yT, bT = res * t2[:nd], res * t2[nd:]
+    blankt = orc.vlor_blankt(g, yT, orc.vlor_precount(g, rec))
+    eD, eT = orc.cylinder(g, 1.0), orc.cylinder(g, 1.0)
+    ptc = orc.forward(g, orc.blur_xy(g, eT, 1.5), 0, 1)
+    dv = np.zeros(nd, np.float32)
+    for s in range(S):
+        proj = orc.forward(g, orc.blur_xy(g, eD, 1.5), s, S)
+        num, den = orc.back(g, orc.ratio(g, proj, yD, bD, s, S), cal, s, S)
+        eD = orc.update(g, eD, num, den, 1.5)
+    for s in range(2):
+        proj = orc.forward(g, orc.blur_xy(g, eT, 1.5), s, S)
+        cur = ptc.reshape(g.nsinogs, g.nang, g.nrad)
+        cur[:, s::S, :] = proj.reshape(g.nsinogs, g.nang, g.nrad)[:, s::S, :]
+        dv = orc.vlor_assign(g, rec, blankt, ptc, s, S, dv)
+        num, den = orc.back(g, orc.ratio(g, proj, dv * res, bT, s, S), cal, s, S)
+        eT = orc.update(g, eT, num, den, 1.5)
+        eT = orc.blend(g, eT, orc.jbf(g, eT, eD, orc.vmax(eD)))
+    close(xD, eD, rtol=0, what="joint c1 xD after 16 doubles passes")
+    close(xT, eT, rtol=0, what="joint c1 xT after 2 triples passes")
+    assert np.array_equal(xD, eD) and np.array_equal(xT, eT)
