@@ -588,83 +588,46 @@ int vlor_create_on(gfr_ctx* ctx, const gfr_geometry* g, const int32_t* rec, long
   *out = nullptr;
   const long long nd = gd->host.info.ndata;
   if (nd >= (1LL << 31)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: sinogram too large");
-  // Valid records (GFIRST.cu:235-237): 1-based indices, 0-based i must satisfy
-  // 0 < i < NDATA for both LORs.
-  std::vector<int> i1, i2;
-  i1.reserve(nrec);
-  i2.reserve(nrec);
-  for (long long e = 0; e < nrec; e++) {
-    const long long a = (long long)rec[2 * e] - 1, b = (long long)rec[2 * e + 1] - 1;
-    if (a <= 0 || b <= 0 || a >= nd || b >= nd) continue;
-    i1.push_back((int)a);
-    i2.push_back((int)b);
-  }
-  const long long nev = (long long)i1.size();
-  // Bin -> record lists in record order: entries (bin, event) in the order
-  // (i1[0], 0), (i2[0], 0), (i1[1], 1), ... sorted by bin with a stable LSD
-  // radix sort (11-bit digits).
-  const size_t ne = 2 * (size_t)nev;
-  std::vector<int> key(ne), val(ne), k2(ne), v2(ne);
-  for (long long e = 0; e < nev; e++) {
-    key[2 * e] = i1[e];
-    key[2 * e + 1] = i2[e];
-    val[2 * e] = val[2 * e + 1] = (int)e;
-  }
-  for (int shift = 0; shift < 31 && (nd - 1) >> shift; shift += 11) {
-    std::vector<size_t> cnt(2049, 0);
-    for (size_t k = 0; k < ne; k++) cnt[((unsigned)key[k] >> shift & 2047u) + 1]++;
-    for (int d = 0; d < 2048; d++) cnt[d + 1] += cnt[d];
-    for (size_t k = 0; k < ne; k++) {
-      const size_t o = cnt[(unsigned)key[k] >> shift & 2047u]++;
-      k2[o] = key[k];
-      v2[o] = val[k];
-    }
-    key.swap(k2);
-    val.swap(v2);
-  }
-  std::vector<long long> bins;
-  std::vector<int> start;
-  for (size_t k = 0; k < ne; k++) {
-    if (k == 0 || key[k] != key[k - 1]) {
-      bins.push_back(key[k]);
-      start.push_back((int)k);
-    }
-  }
-  start.push_back((int)ne);
+  // Valid records (GFIRST.cu:235-237) and the bin -> record lists are built on
+  // the device from the raw records (build_vlor_lists); buffers are sized for
+  // every record being valid.
+  const size_t ne = 2 * (size_t)nrec;
   gfr_vlor* v = new gfr_vlor();
   v->ctx = ctx;
   v->g = *g;
-  v->nev = nev;
-  v->nbins = (long long)bins.size();
   auto dalloc = [&](void** p, size_t b) -> hipError_t {
     v->bytes += b;
     return hipMalloc(p, b ? b : 4);
   };
   float* d_v0 = nullptr;
+  int* d_rec = nullptr;
   hipError_t e = hipSuccess;
-  if (e == hipSuccess) e = dalloc((void**)&v->d_i1, sizeof(int) * nev);
-  if (e == hipSuccess) e = dalloc((void**)&v->d_i2, sizeof(int) * nev);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_i1, sizeof(int) * nrec);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_i2, sizeof(int) * nrec);
   if (e == hipSuccess) e = dalloc((void**)&v->d_blankt, sizeof(float) * nd);
-  if (e == hipSuccess) e = dalloc((void**)&v->d_sinv, sizeof(float) * nev);
-  if (e == hipSuccess) e = dalloc((void**)&v->d_valor, sizeof(float) * nev);
-  if (e == hipSuccess) e = dalloc((void**)&v->d_bins, sizeof(long long) * bins.size());
-  if (e == hipSuccess) e = dalloc((void**)&v->d_start, sizeof(int) * start.size());
+  if (e == hipSuccess) e = dalloc((void**)&v->d_sinv, sizeof(float) * nrec);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_valor, sizeof(float) * nrec);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_bins, sizeof(long long) * ne);
+  if (e == hipSuccess) e = dalloc((void**)&v->d_start, sizeof(int) * (ne + 1));
   if (e == hipSuccess) e = dalloc((void**)&v->d_ev, sizeof(int) * ne);
   if (e == hipSuccess) e = hipMalloc((void**)&d_v0, sizeof(float) * nd);
-  // host sources stay alive until the synchronisation below
-  if (e == hipSuccess && nev) {
-    e = hipMemcpyAsync(v->d_i1, i1.data(), sizeof(int) * nev, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(v->d_i2, i2.data(), sizeof(int) * nev, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(v->d_bins, bins.data(), sizeof(long long) * bins.size(),
-                         hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(v->d_ev, val.data(), sizeof(int) * ne, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && nrec) e = hipMalloc((void**)&d_rec, 2 * sizeof(int) * nrec);
+  if (e == hipSuccess && nrec)
+    e = hipMemcpyAsync(d_rec, rec, 2 * sizeof(int) * nrec, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    long long nev = 0, nbins = 0;
+    e = build_vlor_lists(d_rec, nrec, nd, v->d_i1, v->d_i2, v->d_bins, v->d_start, v->d_ev, &nev,
+                         &nbins, st);
+    v->nev = nev;
+    v->nbins = nbins;
   }
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(v->d_start, start.data(), sizeof(int) * start.size(),
-                       hipMemcpyHostToDevice, st);
+  if (d_rec) (void)hipFree(d_rec);
+  const long long nev = v->nev;
+  if (e == hipSuccess && nrec == 0) {
+    const int zero = 0;
+    e = hipMemcpyAsync(v->d_start, &zero, sizeof(int), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+  }
   // datos_v0 counts on the device (GFIRST.cu:238-239)
   if (e == hipSuccess) e = launch_zero(d_v0, sizeof(float) * nd, st);
   if (e == hipSuccess) e = launch_vlor_count(v->d_i1, v->d_i2, nev, d_v0, st);

@@ -222,6 +222,12 @@ hipError_t launch_vlor_event(const int* i1, const int* i2, const float* blankt,
                              int want_valor, hipStream_t st);
 hipError_t launch_vlor_zero_subset(const DevGeom& G, float* dv, int subset, int nsub,
                                    hipStream_t st);
+// Valid events and bin -> event lists from raw records (device, `d_rec` holds
+// nrec pairs of 1-based bins); sizes: i1/i2 nrec, bins/ev 2*nrec, start
+// 2*nrec+1. Synchronises `st` and returns the counts.
+hipError_t build_vlor_lists(const int* d_rec, long long nrec, long long nd, int* d_i1, int* d_i2,
+                            long long* d_bins, int* d_start, int* d_ev, long long* nev_out,
+                            long long* nbins_out, hipStream_t st);
 hipError_t launch_vlor_bins(const DevGeom& G, const long long* bins, const int* start,
                             const int* ev, long long nbins, const float* proj,
                             const float* sinv, const float* valor, float* dv, int subset,
