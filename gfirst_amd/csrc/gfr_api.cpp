@@ -220,6 +220,8 @@ int gfr_ctx_destroy(gfr_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto& kv : ctx->geoms) {
     if (kv.second->block) (void)hipFree(kv.second->block);
+    if (kv.second->pb_cyl) (void)hipFree(kv.second->pb_cyl);
+    for (auto& pc : kv.second->ptc0) (void)hipFree(pc.second);
     delete kv.second;
   }
   for (auto& kv : ctx->allocs) (void)hipFree(kv.first);

@@ -78,6 +78,14 @@ struct GeomDev {
   DevGeom dev;
   void* block = nullptr;  // single device allocation holding the tables
   std::map<int, Tune> tune;  // per subset count
+  // Sinograms that depend on the geometry alone, computed by the first
+  // session that needs them and then only read: the projection of the
+  // unblurred unit cylinder (PROJ_BLANK, the mask source) and, per forward
+  // PSF (key: its bits), the projection of the blurred unit cylinder (the
+  // joint mode's initial projection cache).
+  std::mutex cache_mu;
+  float* pb_cyl = nullptr;
+  std::map<uint32_t, float*> ptc0;
 };
 
 }  // namespace gfr

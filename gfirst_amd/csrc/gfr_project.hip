@@ -829,7 +829,6 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
   if (out.llh_part && chunk < nav && (chunk & 1)) return hipErrorInvalidValue;
   FwdArgs o{out.proj, out.corr, out.corr2, out.mask, out.y, out.bg, out.yscale, out.llh_part};
   const bool ratio = out.corr || out.corr2;
-  const unsigned nrb = (G.nrad + kWave - 1) / kWave;
   for (int a0 = 0; a0 < nav; a0 += chunk) {
     const int na = min(chunk, nav - a0);
     const unsigned rbytes = (unsigned)((size_t)na * per * sizeof(float));

@@ -81,7 +81,7 @@ Plan plan_of(const GeomHost& h, const gfr_recon_params* p, bool bgD, bool bgT) {
            std::min(nav, (size_t)kRotChunk);
   const bool two = p->mode >= 2;
   pl.nimg = 6 + (two ? 2 : 0);                      // xD num den work xb (+xT jo)
-  pl.nsino_f32 = 1 + (p->mode == 3 ? 2 : 0);        // pb (+ptc dv)
+  pl.nsino_f32 = p->mode == 3 ? 2 : 0;              // ptc dv (PROJ_BLANK is per geometry)
   pl.nsino_store = 1 + (bgD ? 1 : 0) + (two ? 1 + (bgT ? 1 : 0) : 0);
   return pl;
 }
@@ -192,6 +192,53 @@ int rfail(gfr_recon* r, int code) {
 
 extern "C" {
 
+// One of the geometry's shared sinograms (PROJ_BLANK, or with `blurred` the
+// projection of the unit cylinder blurred by `psf`), computed on the
+// session's stream with its scratch buffers by the first session that needs
+// it; later sessions of the geometry only read it.
+hipError_t geom_sino(gfr_recon* r, bool blurred, float psf, const float** out) {
+  GeomDev* gd = r->gd;
+  gfr_ctx* ctx = r->ctx;
+  std::lock_guard<std::mutex> lk(gd->cache_mu);
+  uint32_t key = 0;
+  memcpy(&key, &psf, sizeof(key));
+  float* p = blurred ? (gd->ptc0.count(key) ? gd->ptc0[key] : nullptr) : gd->pb_cyl;
+  if (!p) {
+    const size_t bytes = sizeof(float) * (size_t)gd->host.info.ndata;
+    {
+      std::lock_guard<std::mutex> lc(ctx->mu);
+      if (ctx->budget && ctx->in_use + bytes > ctx->budget) return hipErrorOutOfMemory;
+    }
+    hipError_t e = hipMalloc((void**)&p, bytes);
+    if (e != hipSuccess) return e;
+    const DevGeom& G = gd->dev;
+    const float* src = r->work;
+    e = launch_fill_cylinder(G, 1.f, r->work, r->st);
+    if (e == hipSuccess && blurred) {
+      Taps9 t9;
+      psf_taps(psf, t9.g);
+      e = launch_blur_xy(G, t9, r->work, r->xb, r->st);
+      src = r->xb;
+    }
+    FwdOut fo;
+    fo.proj = p;
+    if (e == hipSuccess) e = launch_forward(G, src, r->R, r->r_floats, fo, 0, 1, nullptr, r->tune, r->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->st);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      return e;
+    }
+    if (blurred)
+      gd->ptc0[key] = p;
+    else
+      gd->pb_cyl = p;
+    std::lock_guard<std::mutex> lc(ctx->mu);
+    ctx->in_use += bytes;
+  }
+  *out = p;
+  return hipSuccess;
+}
+
 long long gfr_recon_footprint(const gfr_geometry* g, const gfr_recon_params* p) {
   GeomHost h;
   if (!p || derive_geometry(g, &h, nullptr) != GFR_OK) return GFR_EINVAL;
@@ -250,7 +297,6 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   bool ok = A((void**)&r->xD, pl.img) && A((void**)&r->num, pl.img) && A((void**)&r->den, pl.img) &&
             A((void**)&r->work, pl.img) && A((void**)&r->xb, pl.img) &&
             A((void**)&r->corr2, pl.corr2) && A((void**)&r->R, pl.rot) &&
-            A((void**)&r->pb, pl.sino_f32) &&
             A((void**)&r->mask, pl.mask) && A((void**)&r->yD, pl.sino_store) &&
             A((void**)&r->llh_part, pl.llh) && A((void**)&r->d_maxbits, 64) &&
             A((void**)&r->d_llh, 64);
@@ -274,12 +320,14 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
     if (ee == hipSuccess) ee = hipStreamSynchronize(st);
     return ee;
   };
-  // Mask source: projection of the unblurred uniform cylinder (GFIRST.cu:155-181)
-  // and the calibration mask (GFIRST.cu:453-458) from the normalization.
-  e = launch_fill_cylinder(G, 1.f, r->xb, st);
-  FwdOut fo;
-  fo.proj = r->pb;
-  if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fo, 0, 1, nullptr, r->tune, st);
+  // Mask source: projection of the unblurred uniform cylinder (GFIRST.cu:155-181,
+  // shared by the geometry's sessions) and the calibration mask
+  // (GFIRST.cu:453-458) from the normalization.
+  {
+    const float* pb = nullptr;
+    e = geom_sino(r, false, 0.f, &pb);
+    r->pb = const_cast<float*>(pb);
+  }
   if (e == hipSuccess) e = hipMemcpyAsync(stage, norm, pl.sino_f32, hipMemcpyDefault, st);
   if (e == hipSuccess) e = launch_calib_mask(G, r->pb, stage, r->mask, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -304,14 +352,12 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
     int rc = vlor_create_on(ctx, g, rec, nrec, r->dv, st, &r->vl);
     if (rc != GFR_OK) return rfail(r, rc);
     // Projection cache of the triples estimate (bins outside the current
-    // subset keep their last projection).
-    Taps9 t9;
-    psf_taps(p->psf_fw, t9.g);
+    // subset keep their last projection): it starts as the projection of the
+    // blurred initial estimate, the unit cylinder (shared per geometry and PSF).
     e = launch_zero(r->dv, pl.sino_f32, st);
-    if (e == hipSuccess) e = launch_blur_xy(G, t9, r->xT, r->xb, st);
-    FwdOut fc;
-    fc.proj = r->ptc;
-    if (e == hipSuccess) e = launch_forward(G, r->xb, r->R, r->r_floats, fc, 0, 1, nullptr, r->tune, st);
+    const float* c0 = nullptr;
+    if (e == hipSuccess) e = geom_sino(r, true, p->psf_fw, &c0);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->ptc, c0, pl.sino_f32, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create vlor cache"));
   }
   e = hipStreamSynchronize(st);
