@@ -919,7 +919,7 @@ def main():
     ap.add_argument("--nsub", type=int, default=16)
     ap.add_argument("--counts", type=float, default=5e7)
     ap.add_argument("--storage", default=None, choices=["f32", "f16", "bf16"])
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--frames", type=int, default=64)
     ap.add_argument("--c3-niter", type=int, default=4)
