@@ -147,10 +147,23 @@ def cpu_baseline(g, d2, norm, pb, nsub, nsteps):
         num, den = orc.back(g, corr, cal, s, nsub)
         x = orc.update(g, x, num, den, 1.5)
     dt = time.perf_counter() - t0
-    return dict(value=nsteps / dt, unit="subset-iters/s", cores=orc.get_threads(), kind="port",
+    cores = orc.get_threads()
+    # the same work on one thread: one subset-iteration
+    orc.set_threads(1)
+    t0 = time.perf_counter()
+    xb = orc.blur_xy(g, x, 1.5)
+    proj = orc.forward(g, xb, 0, nsub)
+    corr = orc.ratio(g, proj, y, None, 0, nsub)
+    num, den = orc.back(g, corr, cal, 0, nsub)
+    orc.update(g, x, num, den, 1.5)
+    d1 = time.perf_counter() - t0
+    orc.set_threads(threads)
+    return dict(value=nsteps / dt, unit="subset-iters/s", cores=cores, kind="port",
                 sample="%d subset-iterations (S=%d) of the same Inveon-scale study, "
                        "oracle FP32 restatement with OpenMP" % (nsteps, nsub),
-                seconds=dt)
+                seconds=dt,
+                single_thread={"value": 1.0 / d1, "cores": 1, "seconds": d1,
+                               "sample": "1 subset-iteration of the same study"})
 
 
 def cpu_baseline_c0(g, d2, norm, nsub, niter, x_gpu):
@@ -166,6 +179,13 @@ def cpu_baseline_c0(g, d2, norm, nsub, niter, x_gpu):
     base = dict(value=niter * nsub / dt, unit="subset-iters/s", cores=orc.get_threads(),
                 kind="port", sample="the whole c0 reconstruction (%d subsets x %d iterations), "
                 "oracle FP32 restatement with OpenMP" % (nsub, niter), seconds=dt)
+    orc.set_threads(1)
+    t0 = time.perf_counter()
+    orc.osem(g, orc.params(mode=1, niter=niter, nsub=nsub), d2, None, norm)
+    d1 = time.perf_counter() - t0
+    orc.set_threads(threads)
+    base["single_thread"] = {"value": niter * nsub / d1, "cores": 1, "seconds": d1,
+                             "sample": "the same reconstruction on one thread"}
     exact = int(np.sum(xD == x_gpu))
     return base, {"bit_exact_voxels": exact, "voxels": int(xD.size)}
 
