@@ -19,7 +19,8 @@ Inveon size: doubles Poisson 5e7 counts from phantom channel 0, triples
 Poisson 1e6 counts from channel 1, 2e6 triple events binned to VLOR records;
 a step is one pass (one subset of one type: doubles, or triples with VLOR
 assignment, JBF and blend), timed over whole iterations. The oracle times one
-pass of each type.
+pass of each type. --c2-mode 2 runs the same study in mode 2 (triples from
+the triples sinogram, no VLOR records).
 
 --workload c3: the batched dynamic study. 64 frames of the same geometry,
 counts log-uniform in [1e5, 1e7], subset counts drawn from {8, 16, 24},
@@ -393,11 +394,11 @@ def cpu_baseline_c2(g, d2, t2, norm, rec, pb, nsub):
     cal = orc.calib(g, pb, norm)
     yD, bD = res * d2[:nd], res * d2[nd:]
     yT, bT = res * t2[:nd], res * t2[nd:]
-    v0 = orc.vlor_precount(g, rec)
-    blankt = orc.vlor_blankt(g, yT, v0)
+    if rec is not None:
+        blankt = orc.vlor_blankt(g, yT, orc.vlor_precount(g, rec))
+        ptc = orc.forward(g, orc.blur_xy(g, orc.cylinder(g, 1.0), 1.5), 0, 1)
     xD = orc.cylinder(g, 1.0)
     xT = orc.cylinder(g, 1.0)
-    ptc = orc.forward(g, orc.blur_xy(g, xT, 1.5), 0, 1)
     dv = np.zeros(nd, np.float32)
     sub = np.zeros((g.nsinogs, g.nang), bool)
     sub[:, 0::nsub] = True
@@ -411,9 +412,12 @@ def cpu_baseline_c2(g, d2, t2, norm, rec, pb, nsub):
     t1 = time.perf_counter()
     xb = orc.blur_xy(g, xT, 1.5)
     proj = orc.forward(g, xb, 0, nsub)
-    ptc[sub] = proj[sub]
-    dv = orc.vlor_assign(g, rec, blankt, ptc, 0, nsub, dv)
-    corr = orc.ratio(g, proj, dv * res, bT, 0, nsub)
+    if rec is not None:
+        ptc[sub] = proj[sub]
+        dv = orc.vlor_assign(g, rec, blankt, ptc, 0, nsub, dv)
+        corr = orc.ratio(g, proj, dv * res, bT, 0, nsub)
+    else:
+        corr = orc.ratio(g, proj, yT, bT, 0, nsub)
     num, den = orc.back(g, corr, cal, 0, nsub)
     xT = orc.update(g, xT, num, den, 1.5)
     filt = orc.jbf(g, xT, xD, orc.vmax(xD))
@@ -421,9 +425,10 @@ def cpu_baseline_c2(g, d2, t2, norm, rec, pb, nsub):
     t2_ = time.perf_counter()
     dt = t2_ - t0
     return dict(value=2 / dt, unit="subset-iters/s", cores=orc.get_threads(), kind="port",
-                sample="one doubles pass (%.1f s) and one triples pass with VLOR assignment, "
-                       "JBF and blend (%.1f s), subset 0 of %d of the same joint study, oracle "
-                       "FP32 restatement with OpenMP" % (t1 - t0, t2_ - t1, nsub),
+                sample="one doubles pass (%.1f s) and one triples pass with %sJBF and blend "
+                       "(%.1f s), subset 0 of %d of the same joint study, oracle FP32 "
+                       "restatement with OpenMP" % (t1 - t0, "VLOR assignment, "
+                                                    if rec is not None else "", t2_ - t1, nsub),
                 seconds=dt)
 
 
@@ -439,8 +444,9 @@ def run_c2(args):
     warm = -(-args.warmup // per_iter) * per_iter
     steps = -(-args.steps // per_iter) * per_iter
     niter = (warm + steps) // per_iter
-    p = ga.recon_params(mode=3, niter=niter, nsub=nsub, storage=storage)
-    r = ctx.recon(g, p, d2, t2, norm, rec)
+    mode = args.c2_mode
+    p = ga.recon_params(mode=mode, niter=niter, nsub=nsub, storage=storage)
+    r = ctx.recon(g, p, d2, t2, norm, rec if mode == 3 else None)
     tune = ctx.tuning(g, nsub)
     log("c2: %d iterations, %d passes each; resident %.2f GB" % (
         niter, per_iter, ctx.bytes_in_use() / 1e9))
@@ -467,7 +473,8 @@ def run_c2(args):
     avg_ms = ph[dom][0] / max(ph[dom][1], 1)
     achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
     res = {
-        "metric": "joint doubles+triples OSEM subset-iters/s",
+        "metric": "joint doubles+triples OSEM subset-iters/s" + (
+            "" if mode == 3 else " (mode 2, no VLOR)"),
         "value": round(steps / dt, 3),
         "unit": "subset-iters/s",
         "n_gpus": 1,
@@ -481,10 +488,12 @@ def run_c2(args):
         "data": "synthetic (seeded mouse phantom channels, device Poisson %.3g doubles / %.3g "
                 "triples counts, %d triple events binned to %d VLOR records, norm U[0.8,1.2])" % (
                     args.counts, args.counts / 50.0, args.events, len(rec)),
-        "config": {"workload": "c2 joint doubles+triples OSEM (mode 3) with VLOR assignment: "
+        "config": {"workload": ("c2 joint doubles+triples OSEM (mode 3) with VLOR assignment: "
+                                if mode == 3 else
+                                "c2 doubles+triples OSEM (mode 2, triples from the sinogram): ") +
                                "image 128x128x159, sinogram 128x160x4319, span 3, MRD 79, "
                                "%d subsets; a step is one pass (one subset of one type)" % nsub,
-                   "subsets": nsub, "storage": args.storage or "f32"},
+                   "subsets": nsub, "storage": args.storage or "f32", "mode": mode},
         "phase_ms_per_iteration": {k: round(v[0] * per_iter / steps, 3) for k, v in ph.items()
                                    if v[1]},
         "roofline": {"kernel": "k_forward" if dom == "forward" else "k_back", "bound": "hbm",
@@ -498,8 +507,8 @@ def run_c2(args):
     r.close()
     if not args.no_cpu:
         log("timing the oracle on one doubles and one triples pass ...")
-        res["cpu_baseline"] = cpu_baseline_c2(g, d2.download(), t2.download(), norm, rec, pb,
-                                              nsub)
+        res["cpu_baseline"] = cpu_baseline_c2(g, d2.download(), t2.download(), norm,
+                                              rec if mode == 3 else None, pb, nsub)
         log("cpu: %.3f subset-iters/s (%s)" % (res["cpu_baseline"]["value"],
                                               res["cpu_baseline"]["sample"]))
     ctx.close()
@@ -953,6 +962,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-active", type=int, default=16)
     ap.add_argument("--events", type=int, default=2000000, help="c2: triple events")
+    ap.add_argument("--c2-mode", type=int, default=3, choices=[2, 3],
+                    help="c2: 3 = triples from VLOR records, 2 = from the triples sinogram")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
