@@ -472,3 +472,23 @@ def test_joint_inveon_first_triples_passes_vs_oracle(gctx, orc):
     close(xD, eD, rtol=0, what="joint c1 xD after 16 doubles passes")
     close(xT, eT, rtol=0, what="joint c1 xT after 2 triples passes")
     assert np.array_equal(xD, eD) and np.array_equal(xT, eT)
+
+
+def test_joint_sessions_share_geometry_sinograms(gctx, orc):
+    """Sessions of one geometry share PROJ_BLANK and, per forward PSF, the
+    initial projection cache of the joint mode: sessions with two PSFs, the
+    first PSF again after the second, each against the oracle."""
+    g = ODD
+    d2, t2, norm, rec = make_study(g, 41, 3)
+    for psf in (1.5, 1.1, 1.5):
+        p = ga.recon_params(mode=3, niter=1, nsub=3, psf_fw=psf)
+        r = gctx.recon(g, p, d2, t2, norm, rec)
+        pb = r.get(ga.OUT_PROJ_BLANK)
+        r.run()
+        xD, xT = r.get(ga.OUT_XD), r.get(ga.OUT_XT)
+        r.close()
+        eD, eT, epb, _ = orc.osem(g, orc.params(mode=3, niter=1, nsub=3, psf_fw=psf), d2, t2,
+                                  norm, rec)
+        assert np.array_equal(pb, epb)
+        close(xD, eD, rtol=0, what="xD psf %.1f" % psf)
+        close(xT, eT, rtol=0, what="xT psf %.1f" % psf)
