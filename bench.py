@@ -98,16 +98,19 @@ def alg_flops(g, nsub):
     return dict(forward=22.0 * L * (g.res + 1), back=10.0 * g.nvox * nviews)
 
 
-def measured_traffic(dom, back_zb, storage, back_lds):
+def measured_traffic(dom, back_zb, storage, back_lds, back_bw=4):
     """HBM bytes per launch of the backprojector from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE runs (scaled by a plain copy's counts),
     for the tile shape in use; None when no measurement matches."""
     path = os.path.join(ROOT, "profiles", "k_back_c1_traffic.json")
-    if dom != "back" or storage != 0 or not os.path.exists(path):
+    if dom != "back" or storage != 0 or back_bw != 4 or not os.path.exists(path):
         return None
     with open(path) as f:
         m = json.load(f)
-    key = ("lds:" if back_lds else "direct:") + str(back_zb)
+    kind = {0: "direct:", 1: "lds:"}.get(back_lds)
+    if kind is None:
+        return None
+    key = kind + str(back_zb)
     e = m.get("by_kind_slices", {}).get(key)
     return None if e is None else e["fetch_bytes"] + e["write_bytes"]
 
@@ -117,7 +120,8 @@ def measured_pipes(dom, tune):
     texture data TD) for the backprojector shape in use, from the committed
     rocprofv3 counter runs; None when no measurement matches."""
     path = os.path.join(ROOT, "profiles", "k_back_c1_pipes.json")
-    if dom != "back" or tune.get("back_lds") or not os.path.exists(path):
+    if dom != "back" or tune.get("back_lds") or tune.get("back_bw", 4) != 4 or \
+            not os.path.exists(path):
         return None
     with open(path) as f:
         m = json.load(f)
@@ -326,7 +330,8 @@ def run_c1(args):
         "roofline": {"kernel": "k_forward" if dom == "forward" else "k_back", "bound": "hbm",
                      "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 5),
-                     "traffic": measured_traffic(dom, tune["back_zb"], storage, tune["back_lds"]),
+                     "traffic": measured_traffic(dom, tune["back_zb"], storage, tune["back_lds"],
+                                                 tune["back_bw"]),
                      "alg_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 4),
                      "fp32_tflops": round(flops, 2),
                      "fp32_frac": round(flops / PEAK_FP32_TFLOPS, 4),

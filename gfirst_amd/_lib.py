@@ -41,8 +41,8 @@ class GfrReconParams(C.Structure):
 
 class GfrTuning(C.Structure):
     _fields_ = [("back_zb", C.c_int), ("back_lds", C.c_int), ("back_waves", C.c_int),
-                ("zsum_sg", C.c_int), ("rot_z", C.c_int), ("ms_back", C.c_float),
-                ("ms_zsum", C.c_float)]
+                ("zsum_sg", C.c_int), ("rot_z", C.c_int), ("back_bw", C.c_int),
+                ("ms_back", C.c_float), ("ms_zsum", C.c_float)]
 
 
 class GfrBatchParams(C.Structure):

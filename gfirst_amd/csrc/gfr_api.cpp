@@ -545,6 +545,7 @@ int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, gfr_tuning* ou
   out->back_waves = t.back_waves;
   out->zsum_sg = t.zsum_sg;
   out->rot_z = t.rot_z;
+  out->back_bw = t.back_bw;
   out->ms_back = t.ms_back;
   out->ms_zsum = t.ms_zsum;
   return t.tuned ? 1 : 0;
@@ -564,6 +565,8 @@ int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, const gfr_tuni
     return set_err(ctx, GFR_EINVAL, "back_waves: 0, or 6 with 8 slices and direct reads");
   if (in->rot_z != 4 && in->rot_z != 16 && in->rot_z != 32)
     return set_err(ctx, GFR_EINVAL, "rot_z must be 4, 16 or 32");
+  if (in->back_bw != 4 && !(in->back_bw == 1 && !in->back_lds && in->back_waves == 0))
+    return set_err(ctx, GFR_EINVAL, "back_bw: 4, or 1 with cached reads and back_waves 0");
   std::lock_guard<std::mutex> lk(ctx->mu);
   Tune& t = gd->tune[nsub];
   t.back_zb = in->back_zb;
@@ -571,6 +574,7 @@ int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, const gfr_tuni
   t.back_waves = in->back_waves;
   t.zsum_sg = in->zsum_sg;
   t.rot_z = in->rot_z;
+  t.back_bw = in->back_bw;
   t.tuned = true;
   t.ms_back = t.ms_zsum = 0.f;
   return GFR_OK;

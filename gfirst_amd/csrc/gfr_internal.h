@@ -51,6 +51,7 @@ struct Tune {
   int back_lds = 0;  // 1: backprojector reads its rows through an LDS window
   int back_waves = 0;     // register budget for this many waves per SIMD (0: compiler's)
   int rot_z = 16;         // planes per rotation block: 4, 16 or 32
+  int back_bw = 4;        // waves per direct backprojector block: 4 (16x16) or 1 (8x8)
   bool tuned = false;
   float ms_back = 0.f, ms_zsum = 0.f;
 };

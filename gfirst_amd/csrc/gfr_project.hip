@@ -690,17 +690,19 @@ __global__ __launch_bounds__(256) void k_back(DevGeom G, const float2* __restric
 // not fit the LDS bounds.
 // MINW > 0 asks the compiler for a register budget that keeps MINW waves per
 // SIMD resident (an autotuner candidate; more resident waves can also thrash
-// the L1).
-template <int ZB, int MINW, class T = float>
-__global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_back_direct(DevGeom G, const float2* __restrict__ c2,
-                                                     unsigned c2_bytes, T* __restrict__ numer,
-                                                     T* __restrict__ denom, int subset,
-                                                     int nsub, const int* __restrict__ dsub) {
+// the L1). A block is BW waves of 8x8 columns: 4 (a 16x16 tile) or 1 (8x8,
+// four times the blocks for small images); each voxel's sum is the same.
+template <int ZB, int MINW, class T = float, int BW = 4>
+__global__ __launch_bounds__(64 * BW, MINW > 0 ? MINW : 1) void k_back_direct(
+    DevGeom G, const float2* __restrict__ c2, unsigned c2_bytes, T* __restrict__ numer,
+    T* __restrict__ denom, int subset, int nsub, const int* __restrict__ dsub) {
+  static_assert(BW == 1 || BW == 4, "blocks of 1 or 4 waves");
+  constexpr int kTile = BW == 4 ? 16 : 8;
   if (dsub) subset = *dsub;
   const int nav = (G.nang - subset + nsub - 1) / nsub;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int ix = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
-  const int iy = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
+  const int ix = blockIdx.x * kTile + (wv & 1) * 8 + (lane & 7);
+  const int iy = blockIdx.y * kTile + (wv >> 1) * 8 + (lane >> 3);
   const int zb = blockIdx.z * ZB;
   const bool act = ix < G.res && iy < G.res;
   const __amdgpu_buffer_rsrc_t rc2 = rsrc(c2, c2_bytes);
@@ -907,6 +909,11 @@ hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* nume
       const unsigned nb = nbx * nbx * (unsigned)((G.nzs + ZB - 1) / ZB);                        \
       hipLaunchKernelGGL(k_back<ZB>, dim3(nb), dim3(256), sizeof(float2) * 2 * rmax * kBW, st,  \
                          G, corr2, (unsigned)bytes, numer, denom, subset, nsub, dsub, rmax);    \
+    } else if (tune.back_bw == 1) {                                                             \
+      const unsigned n8 = (G.res + 7) / 8;                                                      \
+      hipLaunchKernelGGL((k_back_direct<ZB, W, float, 1>), dim3(n8, n8, (G.nzs + ZB - 1) / ZB), \
+                         dim3(64), 0, st, G, corr2, (unsigned)bytes, numer, denom, subset,      \
+                         nsub, dsub);                                                           \
     } else {                                                                                    \
       hipLaunchKernelGGL((k_back_direct<ZB, W>), dim3(nbx, nbx, (G.nzs + ZB - 1) / ZB),         \
                          dim3(256), 0, st, G, corr2, (unsigned)bytes, numer, denom, subset,     \
@@ -985,14 +992,16 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
       t.rot_z = c;
     }
   }
-  // backprojector: (kind, slices, register budget)
-  const int bcand[9][3] = {{0, 4, 0}, {0, 5, 0}, {0, 6, 0}, {0, 8, 0}, {0, 8, 6},
-                           {0, 16, 0}, {1, 4, 0}, {1, 8, 0}, {1, 16, 0}};
+  // backprojector: (kind, slices, register budget, waves per block)
+  const int bcand[12][4] = {{0, 4, 0, 4}, {0, 5, 0, 4}, {0, 6, 0, 4}, {0, 8, 0, 4},
+                            {0, 8, 6, 4}, {0, 16, 0, 4}, {0, 4, 0, 1}, {0, 6, 0, 1},
+                            {0, 8, 0, 1}, {1, 4, 0, 4},  {1, 8, 0, 4}, {1, 16, 0, 4}};
   for (const auto& c : bcand) {
     Tune tt = t;
     tt.back_lds = c[0];
     tt.back_zb = c[1];
     tt.back_waves = c[2];
+    tt.back_bw = c[3];
     const float ms = time_it([&] {
       return launch_back_packed(G, corr2, num, den, 0, nsub, nullptr, tt, st);
     });
@@ -1001,6 +1010,7 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
       t.back_lds = tt.back_lds;
       t.back_zb = tt.back_zb;
       t.back_waves = tt.back_waves;
+      t.back_bw = tt.back_bw;
     }
   }
   (void)hipEventDestroy(e0);

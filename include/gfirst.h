@@ -134,6 +134,8 @@ typedef struct gfr_tuning {
   int back_waves; /* 6: 8-slice direct kernel built for 6 waves/SIMD; else 0     */
   int zsum_sg;    /* sinograms per forward z-sum wave: 4, 6 or 8                 */
   int rot_z;      /* planes per forward rotation block: 4, 16 or 32             */
+  int back_bw;    /* waves per backprojector block: 4 (16x16 columns) or 1 (8x8);
+                     1 only with back_lds 0 and back_waves 0                    */
   float ms_back;  /* measured time of the chosen backprojector (0 if not measured) */
   float ms_zsum;  /* measured time of the chosen forward projector              */
 } gfr_tuning;

@@ -100,12 +100,14 @@ def test_odd_geometry_operators(gctx, orc):
     corr[rng.uniform(size=nd) < 0.05] = -1000.0
     valid = rng.uniform(size=nd) < 0.95
     for s, S in ((0, 1), (4, 5)):
+        ne, de = orc.back(g, corr, np.where(valid, 1, -1000).astype(np.float32), s, S)
         for zb in (4, 5, 6, 8, 16):
-            gctx.set_tuning(g, S, zb, 8)
-            n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
-            ne, de = orc.back(g, corr, np.where(valid, 1, -1000).astype(np.float32), s, S)
-            close(n, ne, rtol=0, what="numer odd zb=%d S=%d" % (zb, S))
-            close(d, de, rtol=0, what="denom odd zb=%d S=%d" % (zb, S))
+            for kind, blk in ((0, 4), (0, 1)):
+                gctx.set_tuning(g, S, zb, 8, back_lds=kind, back_bw=blk)
+                n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
+                what = "odd zb=%d kind=%d blk=%d S=%d" % (zb, kind, blk, S)
+                close(n, ne, rtol=0, what="numer " + what)
+                close(d, de, rtol=0, what="denom " + what)
     close(gctx.blur_xy(g, x, 1.5), orc.blur_xy(g, x, 1.5), rtol=0, what="blur odd")
 
 
@@ -193,18 +195,20 @@ def test_back_project_c1_subset_vs_oracle(gctx, orc):
     close(d2, de, rtol=0, what="denom c1 lds")
 
 
-# back kind: 0 cached reads, 1 per-block LDS row window
-@pytest.mark.parametrize("zb,sg,lds,bw,rz", [(4, 4, 0, 0, 32), (5, 8, 0, 0, 16), (6, 4, 0, 0, 4),
-                                             (8, 6, 0, 0, 4), (16, 8, 0, 0, 16), (8, 8, 0, 6, 4),
-                                             (4, 8, 1, 0, 16), (6, 8, 1, 0, 32), (8, 4, 1, 0, 32),
-                                             (16, 6, 1, 0, 4)])
-def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg, lds, bw, rz):
+# back kind: 0 cached reads, 1 per-block LDS row window; bw: register budget
+# (waves per SIMD); blk: waves per direct backprojector block
+@pytest.mark.parametrize("zb,sg,lds,bw,rz,blk", [
+    (4, 4, 0, 0, 32, 4), (5, 8, 0, 0, 16, 4), (6, 4, 0, 0, 4, 4), (8, 6, 0, 0, 4, 4),
+    (16, 8, 0, 0, 16, 4), (8, 8, 0, 6, 4, 4), (4, 8, 1, 0, 16, 4), (6, 8, 1, 0, 32, 4),
+    (8, 4, 1, 0, 32, 4), (16, 6, 1, 0, 4, 4), (4, 4, 0, 0, 16, 1), (6, 8, 0, 0, 4, 1),
+    (8, 6, 0, 0, 32, 1), (5, 8, 0, 0, 16, 1), (16, 4, 0, 0, 16, 1)])
+def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg, lds, bw, rz, blk):
     """Every autotuner candidate gives the oracle's bits (forward and back)."""
     g = C0
-    gctx.set_tuning(g, 4, zb, sg, lds, bw, rz)
+    gctx.set_tuning(g, 4, zb, sg, lds, bw, rz, back_bw=blk)
     t = gctx.tuning(g, 4)
     assert (t["back_zb"], t["zsum_sg"], t["back_lds"], t["back_waves"], t["rot_z"],
-            t["measured"]) == (zb, sg, lds, bw, rz, True)
+            t["back_bw"], t["measured"]) == (zb, sg, lds, bw, rz, blk, True)
     x = rand_img(g, 30)
     close(gctx.forward_project(g, x, 2, 4), orc.forward(g, x, 2, 4), what="fwd sg=%d" % sg)
     rng = np.random.default_rng(31)
