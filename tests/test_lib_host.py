@@ -28,6 +28,42 @@ def test_header_and_bindings_agree():
     assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
 
 
+def declared_struct(name):
+    """(C type, field name) of `typedef struct name { ... } name;` in the header."""
+    src = open(os.path.join(ROOT, "include", "gfirst.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), src, flags=re.S).group(1)
+    out = []
+    for decl in body.split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        m = re.match(r"(.+?)\s+(\w+(?:\s*,\s*\w+)*)$", decl)
+        out += [(m.group(1), n.strip()) for n in m.group(2).split(",")]
+    return out
+
+
+CTYPE = {"int": C.c_int, "float": C.c_float, "double": C.c_double, "long long": C.c_longlong,
+         "size_t": C.c_size_t}
+
+
+@pytest.mark.parametrize("cname,py", [("gfr_geometry", "GfrGeometry"),
+                                      ("gfr_geometry_info", "GfrGeometryInfo"),
+                                      ("gfr_recon_params", "GfrReconParams"),
+                                      ("gfr_tuning", "GfrTuning"),
+                                      ("gfr_batch_params", "GfrBatchParams"),
+                                      ("gfr_request_info", "GfrRequestInfo"),
+                                      ("gfr_server_totals", "GfrServerTotals")])
+def test_struct_mirrors_match_header(cname, py):
+    """The ctypes mirrors have the header's fields, in order, with its types."""
+    fields = getattr(_lib, py)._fields_
+    decl = declared_struct(cname)
+    assert [n for n, _ in fields] == [n for _, n in decl]
+    for (n, t), (ct, _) in zip(fields, decl):
+        if ct in CTYPE:
+            assert t == CTYPE[ct], (n, ct, t)
+
+
 def test_library_exports_every_declared_symbol():
     L = C.CDLL(_lib.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(L, n)]
