@@ -33,6 +33,8 @@ one study at a time as single sessions.
 for 300 s), 60% c0-size, 30% c1-size, 10% joint doubles+triples (VLOR) at
 c1 size, counts log-uniform in [1e5, 5e7], 2-4 iterations, FP16 storage,
 200 GB HBM budget; p50/p99 latency from submission to the image on the host.
+The process runs with one hardware queue per admitted session
+(GPU_MAX_HW_QUEUES = max_active, at most 32) unless the variable is set.
 
 --copy-probe: a plain 1 GiB device copy (torch) repeated, for scaling the
 profiler's FETCH_SIZE / WRITE_SIZE counters to bytes.
@@ -781,6 +783,10 @@ def c4_inputs(ctx, c, counts, seed):
 
 
 def run_c4(args):
+    # One hardware queue per admitted session (HIP's default is 4 per process,
+    # so sessions would queue behind each other's passes in FIFO order). Read
+    # by the runtime when the first context is created below.
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(args.max_active, 32)))
     import gfirst_amd as ga
     sname = args.storage or "f16"
     storage = STORAGE[sname]
@@ -875,7 +881,8 @@ def run_c4(args):
                                "c0 / 30%% c1 / 10%% c2 (mode 3, 2e6 VLOR events), counts "
                                "log-U[1e5,5e7], 2-4 iterations, %.0f GB budget" % (
                                    args.rate, args.duration, args.budget_gb),
-                   "max_active": args.max_active, "storage": sname},
+                   "max_active": args.max_active, "storage": sname,
+                   "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"])},
         "latency_ms": {"p50": q(allat, 50), "p99": q(allat, 99), "max": q(allat, 100),
                        "per_class": {k: {"n": len(v), "p50": q(v, 50), "p99": q(v, 99)}
                                      for k, v in lat.items()}},
