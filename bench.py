@@ -33,8 +33,8 @@ one study at a time as single sessions.
 for 300 s), 60% c0-size, 30% c1-size, 10% joint doubles+triples (VLOR) at
 c1 size, counts log-uniform in [1e5, 5e7], 2-4 iterations, FP16 storage,
 200 GB HBM budget; p50/p99 latency from submission to the image on the host.
-The process runs with one hardware queue per admitted session
-(GPU_MAX_HW_QUEUES = max_active, at most 32) unless the variable is set.
+The process runs with --hw-queues hardware queues (GPU_MAX_HW_QUEUES, default
+16: one per admitted session; at most 32).
 
 --copy-probe: a plain 1 GiB device copy (torch) repeated, for scaling the
 profiler's FETCH_SIZE / WRITE_SIZE counters to bytes.
@@ -786,7 +786,7 @@ def run_c4(args):
     # One hardware queue per admitted session (HIP's default is 4 per process,
     # so sessions would queue behind each other's passes in FIFO order). Read
     # by the runtime when the first context is created below.
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(args.max_active, 32)))
+    os.environ["GPU_MAX_HW_QUEUES"] = str(max(1, min(args.hw_queues, 32)))
     import gfirst_amd as ga
     sname = args.storage or "f16"
     storage = STORAGE[sname]
@@ -973,6 +973,7 @@ def main():
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--hw-queues", type=int, default=16, help="c4: GPU_MAX_HW_QUEUES (<= 32)")
     ap.add_argument("--events", type=int, default=2000000, help="c2: triple events")
     ap.add_argument("--c2-mode", type=int, default=3, choices=[2, 3],
                     help="c2: 3 = triples from VLOR records, 2 = from the triples sinogram")
