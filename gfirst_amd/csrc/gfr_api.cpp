@@ -590,7 +590,8 @@ int vlor_create_on(gfr_ctx* ctx, const gfr_geometry* g, const int32_t* rec, long
   GFR_NONNULL(ctx, out, "out");
   GFR_NONNULL(ctx, d_yT, "d_yT");
   if (nrec < 0 || (nrec > 0 && !rec)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: records");
-  if (nrec > (1LL << 30)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: too many records");
+  // list offsets are int: 2 * nrec must stay below 2^31
+  if (nrec >= (1LL << 30)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: too many records");
   *out = nullptr;
   const long long nd = gd->host.info.ndata;
   if (nd >= (1LL << 31)) return set_err(ctx, GFR_EINVAL, "gfr_vlor_create: sinogram too large");
