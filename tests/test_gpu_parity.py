@@ -496,3 +496,20 @@ def test_joint_sessions_share_geometry_sinograms(gctx, orc):
         assert np.array_equal(pb, epb)
         close(xD, eD, rtol=0, what="xD psf %.1f" % psf)
         close(xT, eT, rtol=0, what="xT psf %.1f" % psf)
+
+
+def test_tuning_rejects_invalid_shapes(gctx):
+    """gfr_tuning_set validates every field and leaves the shapes in force
+    when it refuses."""
+    g = SMALL
+    gctx.set_tuning(g, 3, 6, 8, back_bw=1)
+    before = gctx.tuning(g, 3)
+    bad = [dict(back_zb=7), dict(zsum_sg=5), dict(rot_z=8), dict(back_lds=2),
+           dict(back_waves=6, back_zb=6), dict(back_waves=6, back_lds=True),
+           dict(back_bw=2), dict(back_bw=1, back_lds=True), dict(back_bw=1, back_waves=6)]
+    for kw in bad:
+        with pytest.raises(ga.GfrError):
+            gctx.set_tuning(g, 3, **kw)
+        assert gctx.tuning(g, 3) == before
+    gctx.set_tuning(g, 3, 8, 8)
+    assert gctx.tuning(g, 3)["back_bw"] == 4
