@@ -16,6 +16,7 @@ GFR_EHIP = -2
 GFR_ENOMEM = -3
 GFR_EIO = -4
 GFR_ESTATE = -5
+GFR_ECANCELED = -6
 
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 OUT_XD, OUT_XT, OUT_PROJ_BLANK, OUT_VLOR = 0, 1, 2, 3
@@ -153,6 +154,7 @@ SIGNATURES = {
     "gfr_server_wait": (_I, [_V, _LL, C.c_double]),
     "gfr_server_result": (_I, [_V, _LL, _V, _V]),
     "gfr_server_release": (_I, [_V, _LL]),
+    "gfr_server_cancel": (_I, [_V, _LL]),
     "gfr_server_stats": (_I, [_V, C.POINTER(GfrServerTotals)]),
     "gfr_server_destroy": (_I, [_V]),
     "gfr_poisson": (_I, [_V, _V, C.c_double, _V, C.c_double, C.c_ulonglong, _V, _LL, _V]),

@@ -595,6 +595,10 @@ class Server:
         self._meta.pop(rid, None)
         self._inputs.pop(rid, None)
 
+    def cancel(self, rid):
+        """Cancel a pending or running request (it ends failed, GFR_ECANCELED)."""
+        _check(self.ctx.h, lib().gfr_server_cancel(self.h, rid))
+
     def stats(self):
         t = _lib.GfrServerTotals()
         _check(self.ctx.h, lib().gfr_server_stats(self.h, C.byref(t)))

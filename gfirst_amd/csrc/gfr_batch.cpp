@@ -348,8 +348,9 @@ int gfr_batch_add(gfr_batch* b, int niter, int nsub, const float* prompts, const
   bool ok = hipMalloc((void**)&fr.x, z.img) == hipSuccess;
   ok = ok && hipMalloc(&fr.y, sb) == hipSuccess;
   if (ok && has_bg) ok = hipMalloc(&fr.bg, sb) == hipSuccess;
-  float* stage = nullptr;
-  if (ok) ok = hipMalloc((void**)&stage, sizeof(float) * (size_t)nd) == hipSuccess;
+  float* stage = nullptr;  // nd floats, then the non-finite flag
+  const size_t nd_al = ((size_t)nd + 63) & ~(size_t)63;
+  if (ok) ok = hipMalloc((void**)&stage, sizeof(float) * nd_al + 256) == hipSuccess;
   auto undo = [&](int code) {
     if (stage) (void)hipFree(stage);
     if (fr.x) (void)hipFree(fr.x);
@@ -364,11 +365,16 @@ int gfr_batch_add(gfr_batch* b, int niter, int nsub, const float* prompts, const
   hipStream_t st = b->st;
   // Stored unscaled in the storage format; kernels scale by RES on load
   // (GFIRST.cu:224-260). Initial estimate: uniform cylinder (GFIRST.cu:282-288).
-  hipError_t e =
-      hipMemcpyAsync(stage, prompts, sizeof(float) * (size_t)nd, hipMemcpyDefault, st);
+  // A NaN or infinite count fails the frame (as gfr_recon_create).
+  unsigned int* bad = (unsigned int*)(stage + nd_al);
+  hipError_t e = launch_zero(bad, sizeof(unsigned int), st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(stage, prompts, sizeof(float) * (size_t)nd, hipMemcpyDefault, st);
+  if (e == hipSuccess) e = launch_any_nonfinite(stage, nd, bad, st);
   if (e == hipSuccess) e = launch_convert_store(stage, fr.y, b->p.storage, nd, st);
   if (e == hipSuccess && has_bg)
     e = hipMemcpyAsync(stage, bg, sizeof(float) * (size_t)nd, hipMemcpyDefault, st);
+  if (e == hipSuccess && has_bg) e = launch_any_nonfinite(stage, nd, bad, st);
   if (e == hipSuccess && has_bg) e = launch_convert_store(stage, fr.bg, b->p.storage, nd, st);
   if (e == hipSuccess) e = launch_fill_cylinder(G, 1.f, fr.x, st);
   const int cur[2] = {0, nsub};  // host source of the two copies, alive until the sync
@@ -376,6 +382,10 @@ int gfr_batch_add(gfr_batch* b, int niter, int nsub, const float* prompts, const
   if (e == hipSuccess) e = hipMemcpyAsync(b->d_nsub + f, &cur[1], sizeof(int), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return undo(hip_err(ctx, e, "gfr_batch_add"));
+  unsigned int hbad = 0;
+  e = hipMemcpy(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return undo(hip_err(ctx, e, "gfr_batch_add"));
+  if (hbad) return undo(set_err(ctx, GFR_EINVAL, "non-finite value in the frame's sinograms"));
   // Tile shapes for this subset count, measured on the first lane's scratch.
   Tune t;
   {

@@ -206,6 +206,9 @@ hipError_t launch_convert_store(const float* in, void* out, int storage, long lo
 hipError_t launch_llh_final(const double* part, int n, double* out, hipStream_t st);
 // *flag = 1 if some v[i] is not <= 0 (NaN counts); *flag must start at 0.
 hipError_t launch_any_positive(const float* v, long long n, unsigned int* flag, hipStream_t st);
+// *flag = 1 if some v[i] is NaN or infinite; *flag must start at 0.
+hipError_t launch_any_nonfinite(const float* v, long long n, unsigned int* flag,
+                                hipStream_t st);
 // sub[slots[i]] = (sub[slots[i]] + 1) mod nsub[slots[i]] for i < n.
 hipError_t launch_advance(int* sub, const int* nsub, const int* slots, int n, hipStream_t st);
 // Synthetic measurements (gfr_synth.hip).

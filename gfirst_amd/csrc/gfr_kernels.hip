@@ -503,7 +503,26 @@ __global__ void k_any_positive(const float* __restrict__ v, long long n,
     any |= !(v[i] <= 0.f);
   if (any) *flag = 1u;
 }
+
+// *flag = 1 if some v[i] is NaN or infinite.
+__global__ void k_any_nonfinite(const float* __restrict__ v, long long n,
+                                unsigned int* __restrict__ flag) {
+  bool bad = false;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    bad |= (__float_as_uint(v[i]) & 0x7f800000u) == 0x7f800000u;
+  if (bad) *flag = 1u;
+}
 }  // namespace
+
+hipError_t launch_any_nonfinite(const float* v, long long n, unsigned int* flag,
+                                hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_any_nonfinite, dim3((unsigned)blocks), dim3(256), 0, st, v, n, flag);
+  return hipGetLastError();
+}
 
 hipError_t launch_any_positive(const float* v, long long n, unsigned int* flag, hipStream_t st) {
   if (n <= 0) return hipSuccess;

@@ -180,7 +180,8 @@ struct FwdArgs {
 // group; otherwise each LOR loads its own two rows. Every LOR still sums its
 // samples in l order with its own weights. Epilogue: ratio
 // (GFIRST.cu:415-430) and the FP64 Poisson log-likelihood partial of the
-// block.
+// block (bins with valid calibration and den >= 0.01: sum of (y/RES)
+// ln(den/RES) - den/RES; the reference computes no likelihood).
 // ---------------------------------------------------------------------------
 template <int S, bool RATIO, bool LLH, int SG>
 __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict__ R,
@@ -281,7 +282,13 @@ __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict
             if (o.corr2)
               store_packed(G, o.corr2, ag, s, rr2, make_float2(c, o.mask[bin] ? 1.f : -1000.f));
             if constexpr (LLH) {
-              if (den > 0.f) ll += (double)yv * log((double)den) - (double)den;
+              // Poisson log-likelihood in count units over the bins whose
+              // ratio enters the backprojection: model mean den/RES, counts
+              // yv/RES (yv and den as the ratio sees them).
+              if (den >= 0.01f && (!o.mask || o.mask[bin])) {
+                const double m = (double)den / (double)o.yscale;
+                ll += ((double)yv / (double)o.yscale) * log(m) - m;
+              }
             }
           }
         }
@@ -315,7 +322,7 @@ __global__ void k_pack(DevGeom G, const float* __restrict__ corr, const uint8_t*
 
 
 // ---------------------------------------------------------------------------
-// Backprojector (BACKWPROJ_GPU, GFIRST_GPU_Kernels.cu:112-160): per voxel and
+// Backprojector (BACKWPROJ_K, GFIRST_GPU_Kernels.cu:112-160): per voxel and
 // view, bilinear (radial bin, plane) interpolation of the packed {ratio,
 // calib} rows, the -1000 gap rule, sums over views in the reference's order
 // (segments outer, angles inner, GFIRST.cu:439-467).

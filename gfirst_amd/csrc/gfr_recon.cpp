@@ -314,8 +314,14 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   float* stage = nullptr;
   if (hipMalloc((void**)&stage, pl.sino_f32) != hipSuccess)
     return rfail(r, set_err(ctx, GFR_ENOMEM, "reconstruction staging allocation failed"));
+  // Measured sinograms with a NaN or infinite value fail the request: the
+  // reference would carry it through the ratio into every voxel the LOR
+  // touches (GFIRST.cu:423-426). The normalization needs no check: a
+  // non-finite value fails its > 0.01 test and masks the bin, as there.
+  unsigned int* bad = r->d_maxbits;
   auto upload = [&](const float* host, void* dst) -> hipError_t {
     hipError_t ee = hipMemcpyAsync(stage, host, pl.sino_f32, hipMemcpyDefault, st);
+    if (ee == hipSuccess) ee = launch_any_nonfinite(stage, nd, bad, st);
     if (ee == hipSuccess) ee = launch_convert_store(stage, dst, p->storage, nd, st);
     if (ee == hipSuccess) ee = hipStreamSynchronize(st);
     return ee;
@@ -332,10 +338,17 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   if (e == hipSuccess) e = launch_calib_mask(G, r->pb, stage, r->mask, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   // Measured data, stored unscaled; kernels scale by RES on load (GFIRST.cu:224-260).
+  if (e == hipSuccess) e = launch_zero(bad, sizeof(unsigned int), st);
   if (e == hipSuccess) e = upload(d2, r->yD);
   if (e == hipSuccess && hasBgD) e = upload(d2 + nd, r->bD);
   if (e == hipSuccess && r->ntypes == 2) e = upload(t2, r->yT);
   if (e == hipSuccess && hasBgT) e = upload(t2 + nd, r->bT);
+  unsigned int hbad = 0;
+  if (e == hipSuccess) e = hipMemcpy(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && hbad) {
+    (void)hipFree(stage);
+    return rfail(r, set_err(ctx, GFR_EINVAL, "non-finite value in the measured sinograms"));
+  }
   // Initial estimates (GFIRST.cu:282-288).
   if (e == hipSuccess) e = launch_fill_cylinder(G, 1.f, r->xD, st);
   if (e == hipSuccess && r->ntypes == 2)

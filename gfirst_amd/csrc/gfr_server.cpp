@@ -5,7 +5,7 @@
 // passes of different requests overlap on the device (continuous batching: a
 // request joins the running set between two passes, it does not wait for the
 // set to drain). The reference reconstructs one study per process run
-// (GFIRST.cu:577-650); a request here is one such run.
+// (GFIRST.cu:577-621); a request here is one such run.
 //
 // Threads: the stepper enqueues passes (never blocks on the device: at most
 // `inflight` passes of a session are queued ahead of the device), the
@@ -36,6 +36,7 @@ struct Req {
   const int32_t* rec = nullptr;
   long long nrec = 0;
   std::atomic<int> state{GFR_REQ_PENDING};
+  std::atomic<bool> cancel{false};  // set by gfr_server_cancel while admitted
   int error = GFR_OK;
   std::string msg;
   double t_submit = 0, t_admit = 0, t_done = 0;
@@ -196,6 +197,14 @@ void stepper(gfr_server* s) {
         q->evs.pop_front();
       }
       if (q->last) {
+        if (q->evs.empty()) finished.push_back(q);
+        continue;
+      }
+      if (q->cancel) {
+        // no further pass; the request retires once its queued passes end
+        q->error = GFR_ECANCELED;
+        q->msg = "request cancelled";
+        q->last = true;
         if (q->evs.empty()) finished.push_back(q);
         continue;
       }
@@ -393,6 +402,31 @@ int gfr_server_release(gfr_server* s, long long id) {
   return GFR_OK;
 }
 
+int gfr_server_cancel(gfr_server* s, long long id) {
+  if (!s) return GFR_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return gfr::set_err(s->ctx, GFR_EINVAL, "unknown request");
+  std::shared_ptr<Req> q = it->second;
+  for (auto pi = s->pending.begin(); pi != s->pending.end(); ++pi) {
+    if (*pi == q) {
+      // not admitted yet: it fails now and never takes device memory
+      s->pending.erase(pi);
+      fail_req(s, q, GFR_ECANCELED, "request cancelled");
+      s->cv_done.notify_all();
+      return GFR_OK;
+    }
+  }
+  const int st = q->state;
+  if (st == GFR_REQ_DONE || st == GFR_REQ_FAILED)
+    return gfr::set_err(s->ctx, GFR_ESTATE, "request already complete");
+  // being created by an admitter, or admitted: the stepper stops it before
+  // its next pass (a request still being created is stopped once admitted)
+  q->cancel = true;
+  s->cv_step.notify_one();
+  return GFR_OK;
+}
+
 int gfr_server_stats(gfr_server* s, gfr_server_totals* st) {
   if (!s || !st) return GFR_EINVAL;
   std::lock_guard<std::mutex> lk(s->mu);
@@ -409,9 +443,13 @@ int gfr_server_stats(gfr_server* s, gfr_server_totals* st) {
 int gfr_server_destroy(gfr_server* s) {
   if (!s) return GFR_EINVAL;
   {
+    // requests never admitted fail now, so that a waiter wakes up
     std::lock_guard<std::mutex> lk(s->mu);
     s->stop = true;
+    for (auto& q : s->pending) fail_req(s, q, GFR_ESTATE, "server destroyed");
+    s->pending.clear();
   }
+  s->cv_done.notify_all();
   s->cv_admit.notify_all();
   s->cv_step.notify_all();
   for (std::thread& t : s->th_admit) t.join();

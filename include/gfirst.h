@@ -39,6 +39,7 @@ extern "C" {
 #define GFR_ENOMEM -3    /* allocation failed or over the HBM budget */
 #define GFR_EIO -4       /* file missing/unreadable                 */
 #define GFR_ESTATE -5    /* call not valid in the object's state     */
+#define GFR_ECANCELED -6 /* request cancelled (gfr_server_cancel)    */
 
 /* Scanner / sinogram / image geometry. The reference compiles these in
  * (GFIRST.h:4-16, 31-34); gfr_geometry_default() returns those values. */
@@ -185,15 +186,15 @@ int gfr_back_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_corr,
                      const uint8_t* d_mask, float* d_numer, float* d_denom, int subset, int nsub,
                      void* stream);
 
-/* Image update (DRV:482-502; GPU:173-188; CPU:165-187): R = DENOM<0.001 ? 1 :
- * NUMER/DENOM; F = Gz(Gxy_psf_bw(R)); x *= F; slice 0 := slice 1 and slice
- * NZS-1 := slice NZS-2. d_work is NVOX floats of scratch. In place on d_x. */
 /* Sensitivity image of a subset: the calibration half of BACKWPROJ
  * (GPU:112-160 with CALIB_PART, DRV:447-467; ratio taken as 1, so only the
  * calibration and segment-edge gaps apply), accumulated over views in FP64.
  * d_sens: NVOX doubles, overwritten. */
 int gfr_sensitivity(gfr_ctx* ctx, const gfr_geometry* g, const uint8_t* d_mask, double* d_sens,
                     int subset, int nsub, void* stream);
+/* Image update (DRV:482-502; GPU:173-188; CPU:165-187): R = DENOM<0.001 ? 1 :
+ * NUMER/DENOM; F = Gz(Gxy_psf_bw(R)); x *= F; slice 0 := slice 1 and slice
+ * NZS-1 := slice NZS-2. d_work is NVOX floats of scratch. In place on d_x. */
 int gfr_update(gfr_ctx* ctx, const gfr_geometry* g, float psf_bw, const float* d_numer,
                const float* d_denom, float* d_x, float* d_work, void* stream);
 
@@ -256,7 +257,10 @@ long long gfr_vlor_bin_hits(const gfr_geometry* g, const float* hits, long long 
  * d2/t2: 2*ndata floats as read from file (unscaled); t2 NULL in mode 1;
  * norm: ndata floats; d2/t2/norm may be host or device memory (device data
  * complete before the call); rec/nrec: VLOR records in host memory (mode 3).
- * Admission checks the context's HBM budget. */
+ * Admission checks the context's HBM budget. A NaN or infinite value in the
+ * measured sinograms (prompts, background) fails with GFR_EINVAL, where the
+ * reference would carry it into the image (DRV:423-426); in the
+ * normalization it masks the bin, as in the reference (DRV:453-458). */
 int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params* p,
                      const float* d2, const float* t2, const float* norm, const int32_t* rec,
                      long long nrec, gfr_recon** out);
@@ -280,7 +284,13 @@ int gfr_recon_cursor(gfr_recon* r, int* iter, int* type, int* subset, long long*
 #define GFR_OUT_PROJ_BLANK 2
 #define GFR_OUT_VLOR 3
 int gfr_recon_get(gfr_recon* r, int what, float* host_out);
-/* FP64 Poisson log-likelihood sum_b (y ln p - p) over the last pass's bins. */
+/* FP64 Poisson log-likelihood of the last pass, in count units: the sum over
+ * the subset's bins whose ratio enters the update (valid calibration and
+ * den = p + max(bg, 0) >= 0.01, DRV:423-429, 453-458) of
+ * (y/RES) ln(den/RES) - den/RES, with y, bg RES-scaled as the ratio sees
+ * them (the constant -ln y! omitted). The reference computes no likelihood
+ * (DRV:413-422 only sums prompts, projection and background). Passes that
+ * fit the VLOR pseudo-sinogram (triples in mode 3) have none: GFR_ESTATE. */
 int gfr_recon_loglik(gfr_recon* r, double* out);
 void* gfr_recon_stream(gfr_recon* r);
 int gfr_recon_destroy(gfr_recon* r);
@@ -326,7 +336,8 @@ int gfr_batch_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_batch_params
                      const float* norm, gfr_batch** out);
 /* Adds a frame: prompts (and background, or NULL) are ndata floats, unscaled,
  * in host or device memory. Admission checks the context's HBM budget
- * (GFR_ENOMEM when it does not fit). Returns the frame id in *frame. */
+ * (GFR_ENOMEM when it does not fit); a NaN or infinite count fails with
+ * GFR_EINVAL. Returns the frame id in *frame. */
 int gfr_batch_add(gfr_batch* b, int niter, int nsub, const float* prompts, const float* bg,
                   int* frame);
 /* One subset pass of every active frame; returns the number of frames still
@@ -338,7 +349,8 @@ int gfr_batch_run(gfr_batch* b);
 int gfr_batch_cursor(gfr_batch* b, int frame, int* iter, int* subset, long long* passes);
 /* Copies a frame's image (nvox floats) to the host. */
 int gfr_batch_get(gfr_batch* b, int frame, float* host_x);
-/* FP64 log-likelihood of the frame's last pass. */
+/* FP64 log-likelihood of the frame's last pass (definition as
+ * gfr_recon_loglik). */
 int gfr_batch_loglik(gfr_batch* b, int frame, double* out);
 /* Frees a frame's slot and memory (waits for the batch's work first). */
 int gfr_batch_remove(gfr_batch* b, int frame);
@@ -362,7 +374,7 @@ int gfr_batch_destroy(gfr_batch* b);
  * passes and their kernels overlap. A request's result is bit-identical to
  * the same session run alone. Host threads inside the server drive the
  * device; the caller only submits and collects. The reference runs one study
- * per process (DRV:577-650). */
+ * per process (DRV:577-621). */
 typedef struct gfr_server gfr_server;
 #define GFR_REQ_PENDING 0
 #define GFR_REQ_ACTIVE 1
@@ -399,9 +411,16 @@ int gfr_server_wait(gfr_server* s, long long id, double timeout_s);
 int gfr_server_result(gfr_server* s, long long id, float* xD, float* xT);
 /* Forgets a done or failed request (frees its host result). */
 int gfr_server_release(gfr_server* s, long long id);
+/* Cancels one request: a pending one fails at once, an admitted one stops
+ * before its next pass and its session is freed; either ends in
+ * GFR_REQ_FAILED with error GFR_ECANCELED while the other requests go on
+ * unchanged. GFR_ESTATE if the request has already completed. (The
+ * reference has no request to cancel; its process runs one study.) */
+int gfr_server_cancel(gfr_server* s, long long id);
 int gfr_server_stats(gfr_server* s, gfr_server_totals* st);
-/* Stops admitting, drops pending requests, waits for and frees running
- * sessions. */
+/* Stops admitting, fails pending requests (GFR_ESTATE, waiters wake up),
+ * waits for and frees running sessions. No other call on the server may
+ * run concurrently with or after this one. */
 int gfr_server_destroy(gfr_server* s);
 
 /* ---------------- synthetic measurements ---------------- */

@@ -167,6 +167,43 @@ void orc_output(const orc_geom* g, float scale, const float* x, float* out) {
       }
 }
 
+/* FP64 Poisson log-likelihood of one (type, subset) pass. The reference
+ * computes none (it only sums prompts, projection and background in float,
+ * GFIRST.cu:413-422); this is the build's definition, stated here as the
+ * checker of the HIP path's reduction:
+ *   LL = sum over the subset's bins b with calib[b] > 0 and
+ *        den_b = p_b + max(bg_b, 0) >= 0.01f (the bins whose ratio enters the
+ *        backprojection, GFIRST.cu:423-429, 453-458)
+ *        of  (y_b/RES) * ln(den_b/RES) - den_b/RES,
+ * i.e. the Poisson log-likelihood (without the constant -ln y!) of the
+ * measured counts under the model mean (p + RES*bg)/RES, in count units.
+ * y and bg are RES-scaled as the ratio sees them (GFIRST.cu:224-260); den is
+ * the ratio's FP32 sum; every other operation is FP64. Bins are summed in
+ * sinogram order; *abs_sum (if not NULL) receives the sum of |term|. */
+double orc_loglik(const orc_geom* g, const float* proj, const float* y, const float* bg,
+                  const float* calib, int subset, int nsub, double* abs_sum) {
+  long long nsino = orc_nsinogs(g);
+  int nang = g->nang, nrad = g->nrad;
+  double res = (double)g->res, ll = 0.0, la = 0.0;
+  for (long long s = 0; s < nsino; s++) {
+    for (int it = subset; it < nang; it += nsub) {
+      size_t b = ((size_t)s * nang + it) * nrad;
+      for (int r = 0; r < nrad; r++) {
+        float vb = bg ? bg[b + r] : 0.f;
+        if (vb < 0.f) vb = 0.f;
+        float den = proj[b + r] + vb;
+        if (!(den >= 0.01f) || !(calib[b + r] > 0.f)) continue;
+        double m = (double)den / res;
+        double t = ((double)y[b + r] / res) * log(m) - m;
+        ll += t;
+        la += fabs(t);
+      }
+    }
+  }
+  if (abs_sum) *abs_sum = la;
+  return ll;
+}
+
 /* ---------------- FP64 error-oracle variant ---------------- */
 #define R double
 #define F(n) orc64_##n

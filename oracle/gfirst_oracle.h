@@ -88,6 +88,11 @@ void orc_vlor_blankt(const orc_geom* g, const float* yT, const float* v0, float*
 void orc_vlor_assign(const orc_geom* g, const int32_t* rec, long long nrec,
                      const float* blankt, const float* proj, float* dv, int subset, int nsub);
 void orc_output(const orc_geom* g, float scale, const float* x, float* out);
+/* FP64 Poisson log-likelihood of a pass over the subset's bins that enter the
+ * update (calib > 0, p + max(bg,0) >= 0.01f): sum (y/RES) ln(den/RES) - den/RES
+ * with y, bg RES-scaled. The reference computes no likelihood. */
+double orc_loglik(const orc_geom* g, const float* proj, const float* y, const float* bg,
+                  const float* calib, int subset, int nsub, double* abs_sum);
 int orc_osem(const orc_geom* g, const orc_params* p, const float* d2, const float* t2,
              const float* norm, const int32_t* rec, long long nrec,
              float* xD, float* xT, float* proj_blank, float* vlor_out);

@@ -86,6 +86,8 @@ def _declare(L):
         "orc_vlor_blankt": (None, [_G, _f32, _f32, _f32]),
         "orc_vlor_assign": (None, [_G, _i32, C.c_longlong, _f32, _f32, _f32, C.c_int, C.c_int]),
         "orc_output": (None, [_G, C.c_float, _f32, _f32]),
+        "orc_loglik": (C.c_double, [_G, _f32, _f32, _opt(_f32), _f32, C.c_int, C.c_int,
+                                    C.POINTER(C.c_double)]),
         "orc_osem": (C.c_int, [_G, _P, _f32, _opt(_f32), _f32, _opt(_i32), C.c_longlong,
                                _f32, _opt(_f32), _opt(_f32), _opt(_f32)]),
         "orc64_angle_tables": (None, [_G, _f64, _f64]),
@@ -286,6 +288,16 @@ def output(g, x, scale):
     out = np.zeros(nvox(g), np.float32)
     lib().orc_output(C.byref(geom(g)), scale, np.ascontiguousarray(x, np.float32), out)
     return out
+
+
+def loglik(g, proj, y, bg, cal, subset=0, nsub=1):
+    """FP64 Poisson log-likelihood of a pass (y, bg RES-scaled; cal the 1/-1000
+    calibration array). Returns (ll, sum of |terms|)."""
+    a = C.c_double(0.0)
+    f = lambda v: np.ascontiguousarray(v, np.float32)
+    ll = lib().orc_loglik(C.byref(geom(g)), f(proj), f(y), None if bg is None else f(bg),
+                          f(cal), subset, nsub, C.byref(a))
+    return float(ll), float(a.value)
 
 
 def osem(g, prm, d2, t2, norm, rec=None, fp64=False, want_vlor=False):

@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 import gfirst_amd as ga
-from gfirst_amd.geometry import C0, SMALL
+from gfirst_amd import synth
+from gfirst_amd.geometry import C0, C1, SMALL
 
 pytestmark = pytest.mark.gpu
 
@@ -166,6 +167,43 @@ def test_device_resident_frame_data_and_budget(gctx):
         bt.add(1, 1, y, b)  # nsub below min_nsub
     bt.close()
     ctx.close()
+
+
+def test_inveon_size_batch_fp16_uneven_subsets(gctx, orc):
+    """The batched dynamic study at its size: three frames on the Inveon
+    geometry (image 128x128x159, sinogram 128x160x4319) with 8, 16 and 24
+    subsets and uneven counts, FP16 storage, one iteration each, advanced
+    together by the captured graph over three lanes. Every frame equals its
+    own session bit for bit (image and FP64 log-likelihood), and the S=8 frame
+    (with a non-integer background) equals the FP32 oracle run on the
+    FP16-rounded inputs: one S=8 iteration, about 15 s on 16 host threads."""
+    g = C1
+    nd = g.ndata
+    norm = synth.norm_uniform(nd, seed=61)
+    spec = [(8, 3e6, True), (16, 1e7, False), (24, 1e5, True)]  # (nsub, counts, bg)
+    bp = ga.batch_params(storage=ga.STORE_F16, max_frames=4, min_nsub=8, lanes=3)
+    bt = gctx.batch(g, bp, norm)
+    data = []
+    for k, (S, counts, hb) in enumerate(spec):
+        y, b = frame_data(g, 900 + k, counts=counts, bg=hb)
+        data.append((y, b))
+        assert bt.add(1, S, y, b) == k
+    bt.run()
+    assert bt.stats()["launches"] == 24
+    for k, (S, _, _) in enumerate(spec):
+        c = bt.cursor(k)
+        assert c["done"] and c["passes"] == S
+        x_ref, ll_ref = session_result(gctx, g, norm, *data[k], 1, S, ga.STORE_F16)
+        same(bt.get(k), x_ref, "c1 frame %d (S=%d)" % (k, S))
+        assert bt.loglik(k) == ll_ref
+    x0 = bt.get(0)
+    bt.close()
+    y, b = data[0]
+    f16 = lambda a: a.astype(np.float16).astype(np.float32)
+    assert not np.array_equal(f16(b), b)
+    d2 = np.concatenate([f16(y), f16(b)])
+    ex, _, _, _ = orc.osem(g, orc.params(mode=1, niter=1, nsub=8), d2, None, norm)
+    same(x0, ex, "c1 frame 0 vs the oracle on FP16-rounded inputs")
 
 
 @pytest.mark.parametrize("lam", [0.05, 0.7, 4.0, 9.99, 10.0, 37.0, 2500.0])

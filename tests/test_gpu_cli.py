@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from gfirst_amd import _lib
-from gfirst_amd.geometry import SMALL
+from gfirst_amd.geometry import REF, SMALL
 
 pytestmark = pytest.mark.gpu
 
@@ -20,8 +20,8 @@ FLAGS = ["--res=%d" % G.res, "--nzs=%d" % G.nzs, "--nrad=%d" % G.nrad, "--nang=%
 SCALE = np.float32(3.70e-4 * 162.0)  # GFIRST.h:29-30
 
 
-def run(cwd, *args):
-    out = subprocess.run([_lib.CLI_PATH] + FLAGS + list(args), cwd=cwd, capture_output=True,
+def run(cwd, *args, flags=FLAGS):
+    out = subprocess.run([_lib.CLI_PATH] + flags + list(args), cwd=cwd, capture_output=True,
                          text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     return out.stdout
@@ -79,3 +79,28 @@ def test_reconstruction_modes(tmp_path, orc, mode):
         assert np.array_equal(raw(tmp_path / "img_GPU_T_ITER2.raw"), orc.output(G, xT, SCALE))
     if mode == 3:
         assert np.array_equal(raw(tmp_path / "SINOG_VLOR.raw"), vo)
+
+
+def test_default_geometry_doubles_reconstruction(tmp_path, orc):
+    """The executable with no geometry flags runs the reference's compiled
+    geometry (GFIRST.h:4-16: 147 x 168 x 1293 sinogram, 128 x 128 x 159 image):
+    one doubles iteration (README mode 2) of 84 two-angle subsets on int32
+    prompts, against the oracle's run of the same files bit for bit. About 10 s
+    of oracle work on 16 host threads."""
+    g = REF
+    nd = g.ndata
+    rng = np.random.default_rng(81)
+    prompts = rng.poisson(rng.uniform(0.05, 0.6, nd)).astype(np.int32)
+    prompts.tofile(tmp_path / "sinogram.raw")
+    norm = rng.uniform(0.8, 1.2, nd).astype(np.float32)
+    norm[rng.uniform(size=nd) < 0.02] = 0.0
+    norm.tofile(tmp_path / "normalization.raw")
+    run(tmp_path, "--niter=1", "--nsub=84", "sinogram.raw", "normalization.raw", flags=[])
+    d2 = np.zeros(2 * nd, np.float32)
+    d2[:nd] = prompts
+    xD, _, pb, _ = orc.osem(g, orc.params(mode=1, niter=1, nsub=84), d2, None, norm)
+    assert np.array_equal(raw(tmp_path / "proj_ref.raw"), pb)
+    got = raw(tmp_path / "img_GPU_D_ITER1.raw")
+    exp = orc.output(g, xD, SCALE)
+    print("ref geometry CLI: %d/%d voxels bit-exact" % (int(np.sum(got == exp)), got.size))
+    assert np.array_equal(got, exp)

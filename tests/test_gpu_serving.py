@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import gfirst_amd as ga
-from gfirst_amd.geometry import C0, SMALL, TINY
+from gfirst_amd.geometry import C0, C1, SMALL, TINY
 
 pytestmark = pytest.mark.gpu
 
@@ -82,6 +82,46 @@ def test_results_alone_equal_results_in_the_mix(gctx):
             assert np.array_equal(xT, eT)
 
 
+def test_inveon_size_requests_in_the_mix(gctx, orc):
+    """The served stream's large classes beside small requests: a c1-size
+    doubles request (Inveon geometry, FP16, 16 subsets) and a c2-size joint
+    request on the same geometry (mode 3, doubles + triples sinograms and
+    4000 VLOR records, FP16), submitted with three c0-size requests. Every
+    result equals the same request alone, bit for bit, and the c1-size
+    doubles image equals the FP32 oracle on the FP16-rounded inputs (one
+    iteration, about 15 s on 16 host threads)."""
+    mix = [(C0, 1, 4, 2, ga.STORE_F16, 1e6), (C1, 1, 16, 1, ga.STORE_F16, 5e7),
+           (C0, 1, 8, 1, ga.STORE_F16, 3e5), (C1, 3, 16, 1, ga.STORE_F16, 2e7),
+           (C0, 1, 4, 1, ga.STORE_F16, 1e5)]
+    srv = gctx.server(max_active=4, inflight=2)
+    ids, reqs = [], []
+    for k, (g, mode, S, ni, st, counts) in enumerate(mix):
+        d2, t2, norm, rec = request(g, 800 + k, mode, counts)
+        if rec is not None:
+            rec = np.random.default_rng(880 + k).integers(1, g.ndata, size=(4000, 2)).astype(
+                np.int32)
+        p = ga.recon_params(mode=mode, niter=ni, nsub=S, storage=st)
+        reqs.append((g, p, d2, t2, norm, rec))
+        ids.append(srv.submit(g, p, d2, t2, norm, rec))
+    for rid in ids:
+        assert srv.wait(rid, 300.0) == ga.REQ_DONE, srv.info(rid)
+    got = [srv.result(rid) for rid in ids]
+    assert srv.stats()["failed"] == 0
+    srv.close()
+    for k, ((g, p, d2, t2, norm, rec), (xD, xT)) in enumerate(zip(reqs, got)):
+        eD, eT = alone(gctx, g, p, d2, t2, norm, rec)
+        n = int(np.sum(xD == eD))
+        print("request %d (%d voxels, mode %d, S=%d): xD %d/%d bit-exact" % (
+            k, g.nvox, p.mode, p.nsub, n, xD.size))
+        assert np.array_equal(xD, eD)
+        if p.mode >= 2:
+            assert np.array_equal(xT, eT)
+    g, p, d2, t2, norm, rec = reqs[1]
+    f16 = lambda a: a.astype(np.float16).astype(np.float32)
+    ex, _, _, _ = orc.osem(g, orc.params(mode=1, niter=1, nsub=16), f16(d2), None, norm)
+    assert np.array_equal(got[1][0], ex)
+
+
 def test_admission_waits_for_budget_and_isolates_failures(gctx):
     g = C0
     p = ga.recon_params(mode=1, niter=1, nsub=4, storage=ga.STORE_F32)
@@ -122,11 +162,64 @@ def test_admission_waits_for_budget_and_isolates_failures(gctx):
     del d2b, nb
 
 
+def test_nan_inputs_and_cancellation_are_isolated(gctx):
+    """Faults forced from the host into a running server: a request with a
+    NaN prompt and one with an infinite background fail with GFR_EINVAL; a
+    request whose normalization holds NaN completes with those bins masked
+    (as the reference's > 0.01 test masks them: the same image as with 0
+    there); a running request and a pending one are cancelled
+    (GFR_ECANCELED). Every other request comes back with its bits alone."""
+    g = C0
+    srv = gctx.server(max_active=2, inflight=2)
+    good = [request(g, 900 + k, 1, 3e5) for k in range(3)]
+    p = ga.recon_params(mode=1, niter=2, nsub=4)
+    long_p = ga.recon_params(mode=1, niter=40, nsub=16)
+    nan_d2 = good[0][0].copy()
+    nan_d2[12345] = np.nan
+    inf_d2 = good[1][0].copy()
+    inf_d2[g.ndata + 77] = np.inf
+    nan_norm = good[2][2].copy()
+    nan_norm[::997] = np.nan
+    ids = {}
+    ids["long"] = srv.submit(g, long_p, good[0][0], None, good[0][2])
+    ids["g0"] = srv.submit(g, p, good[0][0], None, good[0][2])
+    ids["nan"] = srv.submit(g, p, nan_d2, None, good[0][2])
+    ids["inf"] = srv.submit(g, p, inf_d2, None, good[1][2])
+    ids["g1"] = srv.submit(g, p, good[1][0], None, good[1][2])
+    ids["pending"] = srv.submit(g, long_p, good[2][0], None, good[2][2])
+    ids["nannorm"] = srv.submit(g, p, good[2][0], None, nan_norm)
+    srv.cancel(ids["pending"])
+    t0 = time.time()
+    while srv.info(ids["long"])["state"] == ga.REQ_PENDING and time.time() - t0 < 60:
+        time.sleep(0.001)
+    srv.cancel(ids["long"])
+    for rid in ids.values():
+        assert srv.wait(rid, 120.0) in (ga.REQ_DONE, ga.REQ_FAILED)
+    info = {k: srv.info(v) for k, v in ids.items()}
+    for k in ("long", "pending"):
+        assert info[k]["state"] == ga.REQ_FAILED and info[k]["error"] == ga.GFR_ECANCELED, k
+    assert info["pending"]["passes"] == 0
+    assert info["long"]["passes"] < 40 * 16
+    for k in ("nan", "inf"):
+        assert info[k]["state"] == ga.REQ_FAILED and info[k]["error"] == ga.GFR_EINVAL, k
+    with pytest.raises(ga.GfrError) as e:
+        srv.cancel(ids["g0"])  # already complete
+    assert e.value.code == ga.GFR_ESTATE
+    res = {k: srv.result(ids[k])[0] for k in ("g0", "g1", "nannorm")}
+    srv.close()
+    assert np.array_equal(res["g0"], alone(gctx, g, p, good[0][0], None, good[0][2], None)[0])
+    assert np.array_equal(res["g1"], alone(gctx, g, p, good[1][0], None, good[1][2], None)[0])
+    zero_norm = np.where(np.isnan(nan_norm), np.float32(0), nan_norm)
+    exp = alone(gctx, g, p, good[2][0], None, zero_norm, None)[0]
+    assert np.array_equal(res["nannorm"], exp) and np.all(np.isfinite(exp))
+
+
 def test_destroy_with_work_in_flight(gctx):
     srv = gctx.server(max_active=2)
+    ids = []
     for k in range(6):
         d2, _, norm, _ = request(C0, 700 + k, 1, 2e5)
-        srv.submit(C0, ga.recon_params(mode=1, niter=2, nsub=4), d2, None, norm)
+        ids.append(srv.submit(C0, ga.recon_params(mode=1, niter=2, nsub=4), d2, None, norm))
     time.sleep(0.05)
     t0 = time.time()
     srv.close()

@@ -461,6 +461,166 @@ def test_vlor_subset_keeps_other_bins(orc):
     assert np.all(part[it % 2 == 0] == 7.0)
 
 
+
+def vlor_split_f32(rec, bt, proj, nd, swap=False):
+    """The reference's per-record statements (GFIRST.cu:378-387) emulated one
+    float32 operation at a time: valor = BLANKT[i1] + BLANKT[i2]; s = s1 + s2;
+    s_inv = (float)(1.0/s) if s > 0.01f else 0; datos_v[i1] += s1*s_inv*valor,
+    then datos_v[i2] += s2*s_inv*valor. swap=True gives each bin the other
+    bin's share (a plausible mistake the oracle must not make)."""
+    f = np.float32
+    dv = np.zeros(nd, np.float32)
+    for a, b in rec:
+        i1, i2 = int(a) - 1, int(b) - 1
+        if i1 <= 0 or i2 <= 0 or i1 >= nd or i2 >= nd:
+            continue
+        valor = f(bt[i1] + bt[i2])
+        s1, s2 = proj[i1], proj[i2]
+        s = f(s1 + s2)
+        sinv = f(1.0 / float(s)) if s > f(0.01) else f(0.0)
+        w1, w2 = (s2, s1) if swap else (s1, s2)
+        dv[i1] = f(dv[i1] + f(f(w1 * sinv) * valor))
+        dv[i2] = f(dv[i2] + f(f(w2 * sinv) * valor))
+    return dv
+
+
+def test_vlor_assign_split_brute_force(orc):
+    """Hand-built events with unequal projections: every bin's value against
+    the reference's statements (float32 emulation, exact) and a float64 loop
+    (1e-6). Covers the s1:s2 split, an i1 == i2 record, s <= 0.01 (and s
+    exactly 0.01f, which the strict test excludes), index 0, negative and
+    >= NDATA indices, and bins shared by several events in record order."""
+    g = TINY
+    nd = orc.ndata(g)
+    f = np.float32
+    proj = np.zeros(nd, np.float32)
+    bt = np.zeros(nd, np.float32)
+    A, B, Cb, D, E, F, H, K = 110, 125, 140, 177, 200, 300, 400, 501
+    proj[[A, B, Cb, D, E, F, H, K]] = [3.0, 1.0, 0.004, 0.005, 0.5, f(0.01), 0.0, 2.5]
+    bt[[A, B, Cb, D, E, F, H, K]] = [0.75, 2.0, 5.0, 7.0, 1.25, 3.0, 4.0, 0.1]
+    rec = [[A + 1, B + 1],       # s = 4: A takes 3/4, B takes 1/4 of 2.75
+           [Cb + 1, D + 1],      # s = 0.009 <= 0.01: nothing
+           [E + 1, E + 1],       # same bin twice: both halves land on E
+           [F + 1, H + 1],       # s == 0.01f exactly: nothing (strict >)
+           [1, A + 1],           # i1 = 0: skipped (bin 0 is excluded)
+           [0, B + 1],           # i1 = -1: skipped
+           [nd + 1, A + 1],      # i1 = NDATA: skipped
+           [K + 1, A + 1],       # K shares with A: 2.5/5.5 and 3/5.5 of 0.85
+           [B + 1, K + 1]]       # then B and K again, accumulated in record order
+    rng = np.random.default_rng(23)
+    extra = rng.integers(2, 60, size=(40, 2))
+    proj[2:60] = rng.uniform(0.001, 3.0, 58).astype(np.float32)
+    bt[2:60] = rng.uniform(0.0, 2.0, 58).astype(np.float32)
+    rec = np.concatenate([np.array(rec), extra]).astype(np.int32)
+    got = orc.vlor_assign(g, rec, bt, proj)
+    exp = vlor_split_f32(rec, bt, proj, nd)
+    np.testing.assert_array_equal(got, exp)
+    # closed values of the hand-built events
+    assert got[Cb] == 0 and got[D] == 0 and got[F] == 0 and got[H] == 0 and got[0] == 0
+    assert got[E] == f(2.5)
+    # float64 restatement of the same loop
+    d64 = np.zeros(nd)
+    for a, b in rec:
+        i1, i2 = int(a) - 1, int(b) - 1
+        if i1 <= 0 or i2 <= 0 or i1 >= nd or i2 >= nd:
+            continue
+        s = float(proj[i1]) + float(proj[i2])
+        if s > 0.01:
+            v = float(bt[i1]) + float(bt[i2])
+            d64[i1] += proj[i1] / s * v
+            d64[i2] += proj[i2] / s * v
+    np.testing.assert_allclose(got, d64, rtol=1e-6, atol=1e-7)
+    assert abs(got[A] - (0.75 * 2.75 + 3.0 / 5.5 * 0.85)) < 1e-6
+    # the test discriminates: swapped shares and a non-strict 0.01 test differ
+    assert not np.array_equal(vlor_split_f32(rec, bt, proj, nd, swap=True), got)
+    assert got[F] == 0 and float(proj[F]) + float(proj[H]) == float(f(0.01))
+
+
+# ---------------------------------------------------------------- output
+def test_output_scale_and_circular_mask(orc):
+    """Output image (GFIRST.cu:530-549): x * scale_D (float 3.70e-4*162.0)
+    with every voxel zeroed where (x-RES/2)^2 + (y-RES/2)^2 > (RES/2-1)^2;
+    the ring at exactly radius RES/2-1 is kept, the next lattice points go."""
+    for g in (C0, TINY):
+        rng = np.random.default_rng(29)
+        x = rng.uniform(0.1, 9.0, g.nvox).astype(np.float32)
+        scale = np.float32(3.70e-4 * 162.0)
+        out = orc.output(g, x, scale).reshape(g.nzs, g.res, g.res)
+        h = g.res // 2
+        rr = h - 1
+        yy, xx = np.mgrid[0:g.res, 0:g.res]
+        keep = (xx - h) ** 2 + (yy - h) ** 2 <= rr * rr
+        exp = np.where(keep[None], (x * scale).reshape(g.nzs, g.res, g.res), np.float32(0))
+        np.testing.assert_array_equal(out, exp)
+        # on the circle: kept; one step further out: zero
+        for (px, py) in ((h + rr, h), (h - rr, h), (h, h + rr), (h, h - rr)):
+            assert out[1, py, px] == np.float32(x.reshape(out.shape)[1, py, px] * scale)
+        for (px, py) in ((h + rr, h + 1), (h - rr - 1, h), (h, h - rr - 1), (0, 0)):
+            assert out[1, py, px] == 0
+    assert abs(float(scale) - 0.05994) < 1e-8
+
+
+# ---------------------------------------------------------------- log-likelihood
+def test_loglik_closed_form_at_the_data(orc):
+    """Model mean equal to the counts (den = RES*y): LL = sum y ln y - y over
+    the subset's bins with y > 0 (zero bins have den < 0.01 and add 0)."""
+    g = SMALL
+    nd = orc.ndata(g)
+    rng = np.random.default_rng(41)
+    yc = rng.integers(0, 9, nd).astype(np.float32)
+    res = np.float32(g.res)
+    y = res * yc
+    cal = np.ones(nd, np.float32)
+    for s, S in ((0, 1), (2, 3)):
+        ll, la = orc.loglik(g, y, y, None, cal, s, S)
+        it = (np.arange(nd) // g.nrad) % g.nang
+        sel = (it % S == s) & (yc > 0)
+        v = yc[sel].astype(np.float64)
+        exp = math.fsum(v * np.log(v) - v)
+        assert abs(ll - exp) <= 1e-13 * la, (ll, exp)
+
+
+def test_loglik_maximum_exclusions_and_background(orc):
+    """The Poisson LL is largest where the mean equals the counts (any other
+    mean lowers every term); masked bins, bins with den < 0.01 and bins of
+    other subsets do not count; negative background counts as 0 and a
+    positive one enters the mean (y=2, p=RES, bg=RES: 2 ln 2 - 2)."""
+    g = TINY
+    nd = orc.ndata(g)
+    rng = np.random.default_rng(43)
+    res = np.float32(g.res)
+    yc = rng.integers(1, 6, nd).astype(np.float32)
+    y = res * yc
+    cal = np.ones(nd, np.float32)
+    ll0, _ = orc.loglik(g, y, y, None, cal)
+    for k in range(5):
+        p = (y * rng.uniform(0.7, 1.3, nd)).astype(np.float32)
+        assert orc.loglik(g, p, y, None, cal)[0] < ll0
+    # exclusions
+    y2 = y.copy()
+    cal2 = cal.copy()
+    cal2[5] = -1000.0
+    y2[5] = 1e6
+    p2 = y.copy()
+    p2[9] = np.float32(0.005)
+    y2[9] = 1e6
+    a, _ = orc.loglik(g, p2, y2, None, cal2)
+    base, _ = orc.loglik(g, p2, y, None, cal2)
+    assert a == base
+    it = (np.arange(nd) // g.nrad) % g.nang
+    y3 = y.copy()
+    y3[np.flatnonzero(it % 2 == 0)[:7]] = 1e6
+    assert orc.loglik(g, y, y3, None, cal, 1, 2)[0] == orc.loglik(g, y, y, None, cal, 1, 2)[0]
+    # background
+    assert orc.loglik(g, y, y, np.full(nd, -5.0, np.float32), cal)[0] == ll0
+    one = np.zeros(nd, np.float32)
+    one[3] = res
+    cal1 = np.zeros(nd, np.float32) - 1000.0
+    cal1[3] = 1.0
+    ll, _ = orc.loglik(g, one, 2 * one, one, cal1)
+    assert abs(ll - (2 * math.log(2.0) - 2.0)) < 1e-15
+
+
 # ---------------------------------------------------------------- files
 def test_decode_sinogram_int_float_and_truncation(orc):
     nd = 10
