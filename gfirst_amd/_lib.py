@@ -108,6 +108,7 @@ SIGNATURES = {
     "gfr_forward_ratio": (_I, [_V, _G, _V, _V, _V, _V, _V, _I, _I, _V]),
     "gfr_back_project": (_I, [_V, _G, _V, _V, _V, _V, _I, _I, _V]),
     "gfr_update": (_I, [_V, _G, _F, _V, _V, _V, _V, _V]),
+    "gfr_back_project_noalloc": (_I, [_V, _G, _V, _V, _V, _V, _I, _I, _V]),
     "gfr_sensitivity": (_I, [_V, _G, _V, _V, _I, _I, _V]),
     "gfr_osem_run": (_I, [_V, _G, C.POINTER(GfrReconParams), _V, _V, _V, _V, _LL, _V, _V]),
     "gfr_jbf3d": (_I, [_V, _G, _V, _V, _F, _I, _F, _F, _V, _V]),

@@ -198,6 +198,12 @@ class Context:
             return n, d
         return n.download(), d.download()
 
+    def back_project_noalloc(self, g, corr, mask, subset, nsub, numer, denom):
+        """back_project into caller-given DeviceArrays through the library's
+        fixed scratch array (gfr_back_project_noalloc): no device allocation."""
+        _check(self.h, lib().gfr_back_project_noalloc(self.h, C.byref(g.c()), corr.ptr, mask.ptr,
+                                                      numer.ptr, denom.ptr, subset, nsub, None))
+
     def update(self, g, x, numer, denom, psf_bw, work=None):
         """Image update of GFIRST.cu:482-502 (in place on a DeviceArray x)."""
         xd, copied = self._dev(x)

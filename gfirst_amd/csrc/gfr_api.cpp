@@ -83,6 +83,20 @@ void scratch_free(Scratch& s) {
   s = Scratch();
 }
 
+GeomDev* geom_lookup(gfr_ctx* ctx, const gfr_geometry* g) {
+  if (!g) return nullptr;
+  std::string key((const char*)g, sizeof(gfr_geometry));
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  auto it = ctx->geoms.find(key);
+  return it == ctx->geoms.end() ? nullptr : it->second;
+}
+
+Tune tune_lookup(gfr_ctx* ctx, GeomDev* gd, int nsub) {
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  auto it = gd->tune.find(nsub);
+  return it == gd->tune.end() ? Tune() : it->second;
+}
+
 GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g) {
   if (!g) {
     set_err(ctx, GFR_EINVAL, "geometry is NULL");
@@ -198,6 +212,7 @@ int gfr_ctx_create(int device, size_t hbm_budget, gfr_ctx** out) {
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GFR_EHIP;
   if (device < 0 || device >= n) return GFR_EINVAL;
   if (hipSetDevice(device) != hipSuccess) return GFR_EHIP;
+  if (noalloc_init(device) != hipSuccess) return GFR_EHIP;
   gfr_ctx* c = new gfr_ctx();
   c->device = device;
   c->budget = hbm_budget;

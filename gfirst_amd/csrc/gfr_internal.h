@@ -141,6 +141,15 @@ bool is_device_ptr(const void* p);
 
 // Resolves (and caches) the device tables of a geometry.
 GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g);
+// The cached tables of a geometry, or NULL if the context has none yet
+// (allocates nothing).
+GeomDev* geom_lookup(gfr_ctx* ctx, const gfr_geometry* g);
+// The tile shapes in force for (geometry, nsub), defaults if none (no
+// measurement, no allocation).
+Tune tune_lookup(gfr_ctx* ctx, GeomDev* gd, int nsub);
+// Records the address of this device's fixed scratch array of
+// gfr_back_project_noalloc; called with `device` current.
+hipError_t noalloc_init(int device);
 // Tile shapes for (geometry, nsub): measured once per context unless
 // GFR_AUTOTUNE=0 (then the defaults); synchronous on first use.
 Tune tune_for(gfr_ctx* ctx, GeomDev* gd, int nsub);

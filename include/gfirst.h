@@ -186,6 +186,21 @@ int gfr_back_project(gfr_ctx* ctx, const gfr_geometry* g, const float* d_corr,
                      const uint8_t* d_mask, float* d_numer, float* d_denom, int subset, int nsub,
                      void* stream);
 
+/* gfr_back_project without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops): no
+ * counterpart in the reference. The subset's packed rows go to a fixed 64 MiB
+ * device array of the library (one per device, held until the process ends)
+ * instead of context scratch; the kernels and results are gfr_back_project's.
+ * Limits: the packed rows, 8*(NSINOGS + 2*NSEG)*(NRAD + 2) bytes per angle of
+ * the subset, must fit 64 MiB (at the Inveon geometry: 16 or more subsets),
+ * else GFR_EINVAL; the context must already hold the geometry's tables (any
+ * earlier call with this geometry on this context creates them), else
+ * GFR_ESTATE; the context's device must be current. Calls in flight at once
+ * on one device share the array and race: order them on one stream. */
+int gfr_back_project_noalloc(gfr_ctx* ctx, const gfr_geometry* g, const float* d_corr,
+                             const uint8_t* d_mask, float* d_numer, float* d_denom, int subset,
+                             int nsub, void* stream);
+
 /* Sensitivity image of a subset: the calibration half of BACKWPROJ
  * (GPU:112-160 with CALIB_PART, DRV:447-467; ratio taken as 1, so only the
  * calibration and segment-edge gaps apply), accumulated over views in FP64.
