@@ -5,7 +5,11 @@ oblique sinograms (80 rings, span 3, max ring difference 79), 16 subsets,
 FP32, synthetic mouse-like phantom, 5e7 Poisson counts, normalization
 U[0.8, 1.2]. A step is one subset-iteration: PSF blur, forward projection +
 ratio (+ FP64 log-likelihood), backprojection and image update of one
-subset, all on the device with the data resident in HBM.
+subset, all on the device with the data resident in HBM. The same line
+carries the metric's latency half ("serving"): --serve-s seconds (default
+60) of the served stream below (the c4 mix at 4 requests/s, FP16 storage),
+with p50/p99 from submission to the image on the host and the alone-vs-mix
+check of the first request of each class.
 
 --workload c0: the small plain study (64x64x32 image, 64 bins x 64 angles
 x 32 rings, span 3, max ring difference 15; uniform 25 mm cylinder with three
@@ -89,6 +93,16 @@ def alg_bytes(g, nsub, storage_bytes, has_bg):
     update = 4 * nv * 2 + 4 * nv * 2 + 4 * nv * 2  # num+den in, work out; work in, x rw
     return dict(forward=fwd, back=back, blur=blur, update=update,
                 total=fwd + back + blur + update)
+
+
+def step_alg_bytes(g, nsub, storage_bytes, has_bg):
+    """Algorithmic HBM bytes of one subset-iteration by the benchmark plan's
+    formula B_alg = L_s*(b_y + b_bg + b_n + 8) + 16*NVOX: per LOR of the
+    subset the counts, background and normalization read once (b_n = 4: an
+    FP32 normalization value) and the FP32 ratio written and read once; per
+    voxel the image read twice, written once and one intermediate."""
+    L = g.ndata // nsub
+    return L * (storage_bytes * (2 if has_bg else 1) + 4 + 8) + 16 * g.nvox
 
 
 def alg_flops(g, nsub):
@@ -311,8 +325,10 @@ def run_c1(args):
     avg_ms = ph[dom][0] / max(ph[dom][1], 1)
     achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
     flops = af[dom] / (avg_ms * 1e-3) / 1e12
+    step_flops = (af["forward"] + af["back"]) / (ms_step * 1e-3) / 1e12
+    b_step = step_alg_bytes(g, args.nsub, sb, has_bg=False)
     res = {
-        "metric": "Inveon-size OSEM subset-iters/s",
+        "metric": "Inveon-size OSEM subset-iters/s + HBM GB/s; p50/p99 recon latency under load",
         "value": round(args.steps / dt, 3),
         "unit": "subset-iters/s",
         "n_gpus": 1,
@@ -327,22 +343,39 @@ def run_c1(args):
         "config": {"workload": "c1 Inveon-scale doubles OSEM: image 128x128x159, sinogram "
                                "128x160x4319, span 3, MRD 79, %d subsets" % args.nsub,
                    "subsets": args.nsub, "storage": args.storage or "f32"},
-        "hbm_gbps_alg": round(ab["total"] / (ms_step * 1e-3) / 1e9, 2),
+        "hbm_gbps_alg": round(b_step / (ms_step * 1e-3) / 1e9, 2),
+        "hbm_gbps_alg_formula": "B_alg = L_s*(b_y+b_bg+b_n+8) + 16*NVOX = %d bytes per "
+                                "subset-iteration, over ms_per_step" % b_step,
+        "fp32_tflops_step": round(step_flops, 2),
         "phase_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in ph.items() if v[1]},
-        "roofline": {"kernel": "k_forward" if dom == "forward" else "k_back", "bound": "hbm",
-                     "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 5),
+        # The projectors are interpolation gathers: the binding roofline is the
+        # FP32 vector path fed by the CU's vector-memory units (F_alg/t against
+        # 157.3 TFLOP/s); the HBM fraction of the same launch is reported beside.
+        "roofline": {"kernel": "k_zsum+k_rotate_tile" if dom == "forward" else "k_back_direct",
+                     "bound": "fp32-valu (vector-memory fed)",
+                     "achieved": round(flops, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(flops / PEAK_FP32_TFLOPS, 4),
+                     "flops_per_launch": af[dom], "avg_launch_ms": round(avg_ms, 4),
                      "traffic": measured_traffic(dom, tune["back_zb"], storage, tune["back_lds"],
                                                  tune["back_bw"]),
-                     "alg_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 4),
-                     "fp32_tflops": round(flops, 2),
-                     "fp32_frac": round(flops / PEAK_FP32_TFLOPS, 4),
+                     "hbm": {"achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
+                             "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
+                             "alg_bytes_per_launch": ab[dom]},
                      "binding": measured_pipes(dom, tune) if storage == 0 else None},
         "loglik_last_pass": ll,
         "tiles": tune,
     }
     r.close()
     ctx.close()
+    if args.serve_s > 0:
+        # the metric's latency half: a served stream of the c4 mix beside the headline
+        sctx = ga.Context(0, hbm_budget=int(args.budget_gb * 1e9))
+        t = time.perf_counter()
+        cls = c4_setup(sctx, args.seed)
+        log("serve: classes ready in %.1fs" % (time.perf_counter() - t))
+        res["serving"] = served_stream(sctx, cls, STORAGE["f16"], "f16", args.serve_s, args.rate,
+                                       args.max_active, args.seed, args.budget_gb)
+        sctx.close()
     if not args.no_cpu and args.cpu_steps > 0:
         log("timing the CPU oracle on %d subset-iterations ..." % args.cpu_steps)
         res["cpu_baseline"] = cpu_baseline(g, d2, norm, pb, args.nsub, args.cpu_steps)
@@ -782,32 +815,28 @@ def c4_inputs(ctx, c, counts, seed):
     return d2, t2
 
 
-def run_c4(args):
-    # One hardware queue per admitted session (HIP's default is 4 per process,
-    # so sessions would queue behind each other's passes in FIFO order). Read
-    # by the runtime when the first context is created below.
-    os.environ["GPU_MAX_HW_QUEUES"] = str(max(1, min(args.hw_queues, 32)))
+def served_stream(ctx, cls, storage, sname, duration, rate, max_active, seed, budget_gb,
+                  error_report=False):
+    """Plays the served stream: Poisson arrivals at `rate`/s for `duration` s
+    of the c4 mix (classes from c4_setup), one untimed warm-up request per
+    class, then every request waited for; latency = submission to the image
+    on the host. The first request of each class is compared with the same
+    request run alone as a single session (bit for bit)."""
     import gfirst_amd as ga
-    sname = args.storage or "f16"
-    storage = STORAGE[sname]
-    ctx = ga.Context(0, hbm_budget=int(args.budget_gb * 1e9))
-    t = time.perf_counter()
-    cls = c4_setup(ctx, args.seed)
-    log("c4: classes ready in %.1fs" % (time.perf_counter() - t))
-    rng = np.random.default_rng(args.seed + 11)
+    rng = np.random.default_rng(seed + 11)
     arrivals = []
     ta = 0.0
     while True:
-        ta += rng.exponential(1.0 / args.rate)
-        if ta > args.duration:
+        ta += rng.exponential(1.0 / rate)
+        if ta > duration:
             break
         name = rng.choice([c for c, _ in C4_CLASSES], p=[w for _, w in C4_CLASSES])
         counts = float(np.exp(rng.uniform(np.log(1e5), np.log(5e7))))
         niter = int(rng.integers(2, 5))
         arrivals.append((ta, str(name), counts, niter))
-    log("c4: %d arrivals over %.0fs (rate %.1f/s)" % (len(arrivals), args.duration, args.rate))
+    log("serve: %d arrivals over %.0fs (rate %.1f/s)" % (len(arrivals), duration, rate))
     # untimed warm-up: one request of each class (tile shapes, code paths)
-    srv = ctx.server(max_active=args.max_active, inflight=2)
+    srv = ctx.server(max_active=max_active, inflight=2)
     for k, name in enumerate(cls):
         c = cls[name]
         d2, t2 = c4_inputs(ctx, c, 1e6, 10 ** 6 + k)
@@ -822,12 +851,12 @@ def run_c4(args):
         if dt > 0:
             time.sleep(dt)
         c = cls[name]
-        d2, t2 = c4_inputs(ctx, c, counts, args.seed * 7919 + k)
+        d2, t2 = c4_inputs(ctx, c, counts, seed * 7919 + k)
         p = ga.recon_params(mode=c["mode"], niter=niter, nsub=c["nsub"], storage=storage)
         ids.append((srv.submit(c["g"], p, d2, t2, c["norm"], c.get("rec")), name, niter, counts))
         if k % 40 == 0:
             st = srv.stats()
-            log("c4: t=%.1fs submitted %d, active %d, pending %d, done %d" % (
+            log("serve: t=%.1fs submitted %d, active %d, pending %d, done %d" % (
                 time.perf_counter() - t0, k + 1, st["active"], st["pending"], st["done"]))
     for rid, _, _, _ in ids:
         srv.wait(rid)
@@ -855,46 +884,76 @@ def run_c4(args):
             continue
         rid, _, niter, counts = ids[k]
         c = cls[name]
-        d2, t2 = c4_inputs(ctx, c, counts, args.seed * 7919 + k)
+        d2, t2 = c4_inputs(ctx, c, counts, seed * 7919 + k)
         p = ga.recon_params(mode=c["mode"], niter=niter, nsub=c["nsub"], storage=storage)
         r = ctx.recon(c["g"], p, d2, t2, c["norm"], c.get("rec"))
         r.run()
         xD, _ = srv.result(rid)
         same[name] = bool(np.array_equal(r.get(ga.OUT_XD), xD))
         r.close()
-    err = c4_error_report(ctx, srv, cls, storage, args.seed) if args.error_report else None
+    err = c4_error_report(ctx, srv, cls, storage, seed) if error_report else None
     srv.close()
+    out = {
+        "workload": "Poisson arrivals %.1f/s for %.0fs, mix 60%% c0 / 30%% c1 / 10%% c2 "
+                    "(mode 3, 2e6 VLOR events), counts log-U[1e5,5e7], 2-4 iterations, "
+                    "%s storage, %.0f GB budget" % (rate, duration, sname, budget_gb),
+        "p50_ms": q(allat, 50), "p99_ms": q(allat, 99), "max_ms": q(allat, 100),
+        "per_class": {k: {"n": len(v), "p50_ms": q(v, 50), "p99_ms": q(v, 99)}
+                      for k, v in lat.items()},
+        "requests": len(ids), "failed": len(failed),
+        "subset_iters_per_s": round(passes / span, 2),
+        "c1_equiv_subset_iters_per_s": round(eq / span, 2),
+        "makespan_s": round(span, 2),
+        "drain_s": round(t_end - t0 - duration, 2),
+        "alone_equals_mix": same,
+        "max_active": max_active,
+        "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0),
+    }
+    if err is not None:
+        out["error_vs_fp64_oracle"] = err
+    log("serve: p50 %.2f ms, p99 %.2f ms, %d requests, %d failed, alone == mix %s" % (
+        out["p50_ms"], out["p99_ms"], out["requests"], out["failed"], same))
+    return out
+
+
+def run_c4(args):
+    import gfirst_amd as ga  # noqa: F401
+    sname = args.storage or "f16"
+    storage = STORAGE[sname]
+    ctx = ga.Context(0, hbm_budget=int(args.budget_gb * 1e9))
+    t = time.perf_counter()
+    cls = c4_setup(ctx, args.seed)
+    log("c4: classes ready in %.1fs" % (time.perf_counter() - t))
+    sv = served_stream(ctx, cls, storage, sname, args.duration, args.rate, args.max_active,
+                       args.seed, args.budget_gb, args.error_report)
     res = {
         "metric": "p50/p99 recon latency under load",
-        "value": q(allat, 50),
+        "value": sv["p50_ms"],
         "unit": "ms (p50 request latency)",
         "n_gpus": 1,
-        "steps": len(ids),
+        "steps": sv["requests"],
         "warmup": len(cls),
-        "ms_per_step": round(1e3 * span / max(len(ids), 1), 3),
+        "ms_per_step": round(1e3 * sv["makespan_s"] / max(sv["requests"], 1), 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "%s-storage/f32" % sname,
         "data": "synthetic (seeded phantoms, device Poisson, Poisson arrivals)",
-        "config": {"workload": "c4 served stream: Poisson arrivals %.1f/s for %.0fs, mix 60%% "
-                               "c0 / 30%% c1 / 10%% c2 (mode 3, 2e6 VLOR events), counts "
-                               "log-U[1e5,5e7], 2-4 iterations, %.0f GB budget" % (
-                                   args.rate, args.duration, args.budget_gb),
+        "config": {"workload": "c4 served stream: " + sv["workload"],
                    "max_active": args.max_active, "storage": sname,
-                   "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"])},
-        "latency_ms": {"p50": q(allat, 50), "p99": q(allat, 99), "max": q(allat, 100),
-                       "per_class": {k: {"n": len(v), "p50": q(v, 50), "p99": q(v, 99)}
-                                     for k, v in lat.items()}},
-        "requests": len(ids), "failed": len(failed),
-        "subset_iters_per_s": round(passes / span, 2),
-        "c1_equiv_subset_iters_per_s": round(eq / span, 2),
-        "makespan_s": round(span, 2),
-        "drain_s": round(t_end - t0 - args.duration, 2),
-        "alone_equals_mix": same,
+                   "hw_queues": sv["hw_queues"]},
+        "latency_ms": {"p50": sv["p50_ms"], "p99": sv["p99_ms"], "max": sv["max_ms"],
+                       "per_class": {k: {"n": v["n"], "p50": v["p50_ms"], "p99": v["p99_ms"]}
+                                     for k, v in sv["per_class"].items()}},
+        "requests": sv["requests"], "failed": sv["failed"],
+        "subset_iters_per_s": sv["subset_iters_per_s"],
+        "c1_equiv_subset_iters_per_s": sv["c1_equiv_subset_iters_per_s"],
+        "makespan_s": sv["makespan_s"],
+        "drain_s": sv["drain_s"],
+        "alone_equals_mix": sv["alone_equals_mix"],
     }
-    if err is not None:
-        res["error_vs_fp64_oracle"] = err
+    if "error_vs_fp64_oracle" in sv:
+        res["error_vs_fp64_oracle"] = sv["error_vs_fp64_oracle"]
     ctx.close()
     return res
 
@@ -969,7 +1028,10 @@ def main():
     ap.add_argument("--error-report", action="store_true")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true")
-    ap.add_argument("--duration", type=float, default=300.0)
+    ap.add_argument("--duration", type=float, default=300.0, help="c4: stream length, s")
+    ap.add_argument("--serve-s", type=float, default=60.0,
+                    help="c1: length of the served stream reported beside the headline "
+                         "(0: none)")
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-active", type=int, default=16)
@@ -978,6 +1040,12 @@ def main():
     ap.add_argument("--c2-mode", type=int, default=3, choices=[2, 3],
                     help="c2: 3 = triples from VLOR records, 2 = from the triples sinogram")
     args = ap.parse_args()
+    if args.workload in ("c1", "c4"):
+        # One hardware queue per admitted session of the served stream (HIP's
+        # default is 4 per process, so sessions would queue behind each
+        # other's passes in FIFO order); read by the runtime when the first
+        # context is created.
+        os.environ["GPU_MAX_HW_QUEUES"] = str(max(1, min(args.hw_queues, 32)))
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
