@@ -509,9 +509,11 @@ def run_c2(args):
         steps, ms_step, {k: round(v[0], 2) for k, v in ph.items()}))
     sb = 4 if storage == ga.STORE_F32 else 2
     ab = alg_bytes(g, nsub, sb, has_bg=False)
+    af = alg_flops(g, nsub)
     dom = max(("forward", "back"), key=lambda k: ph[k][0])
     avg_ms = ph[dom][0] / max(ph[dom][1], 1)
     achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
+    flops = af[dom] / (avg_ms * 1e-3) / 1e12
     res = {
         "metric": "joint doubles+triples OSEM subset-iters/s" + (
             "" if mode == 3 else " (mode 2, no VLOR)"),
@@ -536,10 +538,16 @@ def run_c2(args):
                    "subsets": nsub, "storage": args.storage or "f32", "mode": mode},
         "phase_ms_per_iteration": {k: round(v[0] * per_iter / steps, 3) for k, v in ph.items()
                                    if v[1]},
-        "roofline": {"kernel": "k_forward" if dom == "forward" else "k_back", "bound": "hbm",
-                     "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
-                     "alg_bytes_per_launch": ab[dom], "avg_launch_ms": round(avg_ms, 4)},
+        # phases of the two chains overlap on the device: a phase's event
+        # time includes the other chain's kernels running beside it
+        "roofline": {"kernel": "k_zsum+k_rotate_tile" if dom == "forward" else "k_back_direct",
+                     "bound": "fp32-valu (vector-memory fed)",
+                     "achieved": round(flops, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(flops / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "flops_per_launch": af[dom], "avg_launch_ms": round(avg_ms, 4),
+                     "hbm": {"achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
+                             "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
+                             "alg_bytes_per_launch": ab[dom]}},
         "cursor_after": cur,
         "tiles": tune,
     }
@@ -1040,11 +1048,11 @@ def main():
     ap.add_argument("--c2-mode", type=int, default=3, choices=[2, 3],
                     help="c2: 3 = triples from VLOR records, 2 = from the triples sinogram")
     args = ap.parse_args()
-    if args.workload in ("c1", "c4"):
-        # One hardware queue per admitted session of the served stream (HIP's
-        # default is 4 per process, so sessions would queue behind each
-        # other's passes in FIFO order); read by the runtime when the first
-        # context is created.
+    if args.workload in ("c1", "c2", "c4"):
+        # One hardware queue per stream in use: per admitted session of the
+        # served stream, and the two chains of a joint session (HIP's default
+        # is 4 per process, and streams sharing a queue run in FIFO order);
+        # read by the runtime when the first context is created.
         os.environ["GPU_MAX_HW_QUEUES"] = str(max(1, min(args.hw_queues, 32)))
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

@@ -130,6 +130,7 @@ SIGNATURES = {
                               C.POINTER(_LL)]),
     "gfr_recon_get": (_I, [_V, _I, _V]),
     "gfr_recon_loglik": (_I, [_V, C.POINTER(C.c_double)]),
+    "gfr_recon_sync": (_I, [_V]),
     "gfr_recon_stream": (_V, [_V]),
     "gfr_recon_destroy": (_I, [_V]),
     "gfr_recon_profile": (_I, [_V, _I]),

@@ -407,7 +407,8 @@ class Recon:
         return lib().gfr_recon_stream(self.h)
 
     def sync(self):
-        self.ctx.sync(self.stream)
+        """Wait for every pass enqueued so far (both chains of a joint session)."""
+        _check(self.ctx.h, lib().gfr_recon_sync(self.h))
 
     def close(self):
         if self.h:

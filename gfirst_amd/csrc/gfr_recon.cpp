@@ -5,6 +5,16 @@
 //
 // Loop nest: iter -> type (doubles, then triples) -> subset. Subset s holds
 // the angles theta with theta % S == s; S = 1 is the reference's loop.
+//
+// Joint modes (2, 3) run two chains. The triples passes of iteration i read
+// the doubles image only as the bilateral filter's guide (and its maximum),
+// and the doubles passes of iteration i+1 do not read the triples image, so
+// the triples passes go to a second stream with scratch of their own, guided
+// by a snapshot of the doubles image taken when iteration i's doubles passes
+// end; they run beside iteration i+1's doubles passes (the bilateral filter
+// is bound by VALU issue, the projectors by the vector-memory path). Every
+// pass sees exactly the inputs of the reference's sequential loop
+// (GFIRST.cu:303-527), so the images are the same bits.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -38,6 +48,17 @@ struct gfr_recon {
   double* d_llh = nullptr;
   unsigned int* d_maxbits = nullptr;
   int llh_valid = 0;
+  // triples chain (modes 2-3): stream, scratch, the doubles snapshot
+  hipStream_t stT = nullptr;
+  float *xbT = nullptr, *numT = nullptr, *denT = nullptr, *workT = nullptr, *xDs = nullptr;
+  float2* corr2T = nullptr;
+  float* RT = nullptr;
+  double *llh_partT = nullptr, *d_llhT = nullptr;
+  hipEvent_t ev_snap = nullptr;   // snapshot of the doubles image taken
+  hipEvent_t ev_tfree = nullptr;  // an iteration's triples passes are done with it
+  hipEvent_t ev_join = nullptr;
+  bool tfree_pending = false;
+  int llh_chain = 0;  // chain of the last pass's log-likelihood (0 doubles, 1 triples)
   std::vector<void*> owned;
   size_t bytes = 0;
   Tune tune;
@@ -59,6 +80,7 @@ size_t store_bytes(int storage) { return storage == GFR_STORE_F32 ? 4 : 2; }
 struct Plan {
   size_t img, sino_f32, sino_store, mask, llh, corr2, rot;
   int nimg, nsino_f32, nsino_store;
+  bool two;  // second (triples) chain
 };
 
 bool any_positive(const float* v, long long n) {
@@ -80,7 +102,10 @@ Plan plan_of(const GeomHost& h, const gfr_recon_params* p, bool bgD, bool bgT) {
   pl.rot = sizeof(float) * (size_t)(h.info.lambda1 - h.info.lambda0 + 1) * h.g.nzs * h.g.nrad *
            std::min(nav, (size_t)kRotChunk);
   const bool two = p->mode >= 2;
-  pl.nimg = 6 + (two ? 2 : 0);                      // xD num den work xb (+xT jo)
+  pl.two = two;
+  // xD num den work xb (+ xT jo, and the triples chain's xb num den work and
+  // the doubles snapshot)
+  pl.nimg = 6 + (two ? 7 : 0);
   pl.nsino_f32 = p->mode == 3 ? 2 : 0;              // ptc dv (PROJ_BLANK is per geometry)
   pl.nsino_store = 1 + (bgD ? 1 : 0) + (two ? 1 + (bgT ? 1 : 0) : 0);
   return pl;
@@ -88,7 +113,7 @@ Plan plan_of(const GeomHost& h, const gfr_recon_params* p, bool bgD, bool bgT) {
 
 size_t plan_bytes(const Plan& pl) {
   return pl.nimg * pl.img + pl.nsino_f32 * pl.sino_f32 + pl.nsino_store * pl.sino_store +
-         pl.mask + pl.llh + pl.corr2 + pl.rot;
+         pl.mask + (pl.two ? 2 : 1) * (pl.llh + pl.corr2 + pl.rot);
 }
 
 hipEvent_t take_event(gfr_recon* r) {
@@ -102,21 +127,23 @@ hipEvent_t take_event(gfr_recon* r) {
   return e;
 }
 
-// Brackets one phase of a pass with events when tracing is on.
+// Brackets one phase of a pass (on the pass's stream) with events when
+// tracing is on.
 struct PhaseMark {
   gfr_recon* r;
   int phase;
+  hipStream_t st;
   hipEvent_t e0 = nullptr;
-  PhaseMark(gfr_recon* rr, int ph) : r(rr), phase(ph) {
+  PhaseMark(gfr_recon* rr, int ph, hipStream_t s) : r(rr), phase(ph), st(s) {
     if (r->prof) {
       e0 = take_event(r);
-      if (e0) (void)hipEventRecord(e0, r->st);
+      if (e0) (void)hipEventRecord(e0, st);
     }
   }
   void end() {
     if (!e0) return;
     hipEvent_t e1 = take_event(r);
-    if (e1 && hipEventRecord(e1, r->st) == hipSuccess) {
+    if (e1 && hipEventRecord(e1, st) == hipSuccess) {
       r->ev_pending.push_back({phase, {e0, e1}});
     } else {
       r->ev_free.push_back(e0);
@@ -303,7 +330,16 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   if (ok && hasBgD) ok = A(&r->bD, pl.sino_store);
   if (ok && r->ntypes == 2)
     ok = A((void**)&r->xT, pl.img) && A((void**)&r->jo, pl.img) && A(&r->yT, pl.sino_store) &&
-         (!hasBgT || A(&r->bT, pl.sino_store));
+         (!hasBgT || A(&r->bT, pl.sino_store)) && A((void**)&r->xbT, pl.img) &&
+         A((void**)&r->numT, pl.img) && A((void**)&r->denT, pl.img) &&
+         A((void**)&r->workT, pl.img) && A((void**)&r->xDs, pl.img) &&
+         A((void**)&r->corr2T, pl.corr2) && A((void**)&r->RT, pl.rot) && A((void**)&r->llh_partT, pl.llh) &&
+         A((void**)&r->d_llhT, 64);
+  if (ok && r->ntypes == 2)
+    ok = hipStreamCreateWithFlags(&r->stT, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&r->ev_snap, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&r->ev_tfree, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) == hipSuccess;
   if (ok && p->mode == 3) ok = A((void**)&r->ptc, pl.sino_f32) && A((void**)&r->dv, pl.sino_f32);
   if (!ok) return rfail(r, set_err(ctx, GFR_ENOMEM, "reconstruction allocation failed"));
   hipStream_t st = r->st;
@@ -387,9 +423,29 @@ int gfr_recon_step(gfr_recon* r) {
   gfr_ctx* ctx = r->ctx;
   const DevGeom& G = r->gd->dev;
   const gfr_recon_params& p = r->p;
-  hipStream_t st = r->st;
   const int ty = r->type, s = r->subset, S = p.nsub;
-  float* x = ty == 0 ? r->xD : r->xT;
+  // the pass's chain: doubles on the session's stream, triples on the second
+  const bool tc = ty == 1;
+  hipStream_t st = tc ? r->stT : r->st;
+  float* x = tc ? r->xT : r->xD;
+  float* xb = tc ? r->xbT : r->xb;
+  float* R = tc ? r->RT : r->R;
+  float2* corr2 = tc ? r->corr2T : r->corr2;
+  float* num = tc ? r->numT : r->num;
+  float* den = tc ? r->denT : r->den;
+  float* work = tc ? r->workT : r->work;
+  double* llh_part = tc ? r->llh_partT : r->llh_part;
+  double* d_llh = tc ? r->d_llhT : r->d_llh;
+  if (tc && s == 0) {
+    // Fork: the doubles image at the end of this iteration's doubles passes
+    // guides this iteration's triples passes; the previous iteration's
+    // triples passes must be done with the snapshot first.
+    if (r->tfree_pending) GFR_HIP(ctx, hipStreamWaitEvent(r->st, r->ev_tfree, 0));
+    GFR_HIP(ctx, hipMemcpyAsync(r->xDs, r->xD, sizeof(float) * (size_t)r->gd->host.info.nvox,
+                                hipMemcpyDeviceToDevice, r->st));
+    GFR_HIP(ctx, hipEventRecord(r->ev_snap, r->st));
+    GFR_HIP(ctx, hipStreamWaitEvent(r->stT, r->ev_snap, 0));
+  }
   Taps9 tf, tb;
   Taps5 tz;
   psf_taps(p.psf_fw, tf.g);
@@ -397,58 +453,63 @@ int gfr_recon_step(gfr_recon* r) {
   gz_taps(tz.g);
   // PSF blur before projection (GFIRST.cu:324-340).
   {
-    PhaseMark pm(r, GFR_PHASE_BLUR);
-    GFR_HIP(ctx, launch_blur_xy(G, tf, x, r->xb, st));
+    PhaseMark pm(r, GFR_PHASE_BLUR, st);
+    GFR_HIP(ctx, launch_blur_xy(G, tf, x, xb, st));
   }
-  if (ty == 1 && p.mode == 3) {
+  if (tc && p.mode == 3) {
     // Forward projection into the cache, VLOR reassignment of the subset's bins
     // (GFIRST.cu:373-397), ratio against the pseudo-sinogram x RES.
     {
-      PhaseMark pm(r, GFR_PHASE_FORWARD);
+      PhaseMark pm(r, GFR_PHASE_FORWARD, st);
       FwdOut fc;
       fc.proj = r->ptc;
-      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fc, s, S, nullptr, r->tune, st));
+      GFR_HIP(ctx, launch_forward(G, xb, R, r->r_floats, fc, s, S, nullptr, r->tune, st));
     }
-    PhaseMark pm(r, GFR_PHASE_VLOR);
+    PhaseMark pm(r, GFR_PHASE_VLOR, st);
     int rc = gfr_vlor_assign(ctx, r->vl, r->ptc, r->dv, s, S, st);
     if (rc != GFR_OK) return rc;
     GFR_HIP(ctx, launch_ratio(G, r->ptc, r->dv, r->bT, p.storage, (float)r->g.res, nullptr,
-                              r->corr2, r->mask, s, S, st));
+                              corr2, r->mask, s, S, st));
     r->llh_valid = 0;
   } else {
-    const void* y = ty == 0 ? r->yD : r->yT;
-    const void* bg = ty == 0 ? r->bD : r->bT;
+    const void* y = tc ? r->yT : r->yD;
+    const void* bg = tc ? r->bT : r->bD;
     {
-      PhaseMark pm(r, GFR_PHASE_FORWARD);
+      PhaseMark pm(r, GFR_PHASE_FORWARD, st);
       FwdOut fo;
-      fo.corr2 = r->corr2;
+      fo.corr2 = corr2;
       fo.mask = r->mask;
       fo.y = y;
       fo.bg = bg;
       fo.storage = p.storage;
       fo.yscale = (float)r->g.res;
-      fo.llh_part = r->llh_part;
-      GFR_HIP(ctx, launch_forward(G, r->xb, r->R, r->r_floats, fo, s, S, nullptr, r->tune, st));
+      fo.llh_part = llh_part;
+      GFR_HIP(ctx, launch_forward(G, xb, R, r->r_floats, fo, s, S, nullptr, r->tune, st));
     }
-    GFR_HIP(ctx, launch_llh_sum(G, r->llh_part, s, S, nullptr, r->tune.zsum_sg, r->d_llh, st));
+    GFR_HIP(ctx, launch_llh_sum(G, llh_part, s, S, nullptr, r->tune.zsum_sg, d_llh, st));
     r->llh_valid = 1;
+    r->llh_chain = tc ? 1 : 0;
   }
   // Backprojection of the subset (GFIRST.cu:436-469) and update (:482-502).
   {
-    PhaseMark pm(r, GFR_PHASE_BACK);
-    GFR_HIP(ctx, launch_back_packed(G, r->corr2, r->num, r->den, s, S, nullptr, r->tune, st));
+    PhaseMark pm(r, GFR_PHASE_BACK, st);
+    GFR_HIP(ctx, launch_back_packed(G, corr2, num, den, s, S, nullptr, r->tune, st));
   }
   {
-    PhaseMark pm(r, GFR_PHASE_UPDATE);
-    GFR_HIP(ctx, launch_update(G, tb, tz, r->num, r->den, x, r->work, st));
+    PhaseMark pm(r, GFR_PHASE_UPDATE, st);
+    GFR_HIP(ctx, launch_update(G, tb, tz, num, den, x, work, st));
   }
-  if (ty == 1) {
+  if (tc) {
     // Joint bilateral filter of triples guided by doubles + blend (:504-526).
-    PhaseMark pm(r, GFR_PHASE_JBF);
-    GFR_HIP(ctx, launch_max(G, r->xD, r->d_maxbits, st));
-    GFR_HIP(ctx, launch_jbf(G, r->xT, r->xD, (const float*)r->d_maxbits, 0.f, p.jbf_r,
+    PhaseMark pm(r, GFR_PHASE_JBF, st);
+    GFR_HIP(ctx, launch_max(G, r->xDs, r->d_maxbits, st));
+    GFR_HIP(ctx, launch_jbf(G, r->xT, r->xDs, (const float*)r->d_maxbits, 0.f, p.jbf_r,
                             p.jbf_sigma3d, p.jbf_sigmai, r->jo, st));
     GFR_HIP(ctx, launch_blend(G, p.jbf_coef, r->jo, r->xT, st));
+    if (s == S - 1) {
+      GFR_HIP(ctx, hipEventRecord(r->ev_tfree, st));
+      r->tfree_pending = true;
+    }
   }
   r->passes++;
   if (++r->subset == S) {
@@ -458,8 +519,22 @@ int gfr_recon_step(gfr_recon* r) {
       if (++r->iter > p.niter) r->done = true;
     }
   }
+  if (r->done && r->stT) {
+    // join: the session's stream completes after the triples chain
+    GFR_HIP(ctx, hipEventRecord(r->ev_join, r->stT));
+    GFR_HIP(ctx, hipStreamWaitEvent(r->st, r->ev_join, 0));
+  }
   return r->done ? 1 : 0;
 }
+
+namespace {
+// Waits for both chains of a session.
+hipError_t sync_all(gfr_recon* r) {
+  hipError_t e = r->stT ? hipStreamSynchronize(r->stT) : hipSuccess;
+  if (e == hipSuccess) e = hipStreamSynchronize(r->st);
+  return e;
+}
+}  // namespace
 
 int gfr_recon_run(gfr_recon* r) {
   if (!r) return GFR_EINVAL;
@@ -467,7 +542,7 @@ int gfr_recon_run(gfr_recon* r) {
   while ((rc = gfr_recon_step(r)) == 0) {
   }
   if (rc < 0) return rc;
-  GFR_HIP(r->ctx, hipStreamSynchronize(r->st));
+  GFR_HIP(r->ctx, sync_all(r));
   return GFR_OK;
 }
 
@@ -493,6 +568,7 @@ int gfr_recon_get(gfr_recon* r, int what, float* host_out) {
     default: return set_err(r->ctx, GFR_EINVAL, "gfr_recon_get: unknown output");
   }
   if (!src) return set_err(r->ctx, GFR_ESTATE, "gfr_recon_get: output not held in this mode");
+  GFR_HIP(r->ctx, sync_all(r));
   GFR_HIP(r->ctx, hipMemcpyAsync(host_out, src, bytes, hipMemcpyDeviceToHost, r->st));
   GFR_HIP(r->ctx, hipStreamSynchronize(r->st));
   return GFR_OK;
@@ -501,12 +577,20 @@ int gfr_recon_get(gfr_recon* r, int what, float* host_out) {
 int gfr_recon_loglik(gfr_recon* r, double* out) {
   if (!r || !out) return GFR_EINVAL;
   if (!r->llh_valid) return set_err(r->ctx, GFR_ESTATE, "no log-likelihood for the last pass");
-  GFR_HIP(r->ctx, hipMemcpyAsync(out, r->d_llh, sizeof(double), hipMemcpyDeviceToHost, r->st));
+  GFR_HIP(r->ctx, sync_all(r));
+  GFR_HIP(r->ctx, hipMemcpyAsync(out, r->llh_chain ? r->d_llhT : r->d_llh, sizeof(double),
+                                 hipMemcpyDeviceToHost, r->st));
   GFR_HIP(r->ctx, hipStreamSynchronize(r->st));
   return GFR_OK;
 }
 
 void* gfr_recon_stream(gfr_recon* r) { return r ? (void*)r->st : nullptr; }
+
+int gfr_recon_sync(gfr_recon* r) {
+  if (!r) return GFR_EINVAL;
+  GFR_HIP(r->ctx, sync_all(r));
+  return GFR_OK;
+}
 
 int gfr_recon_profile(gfr_recon* r, int enable) {
   if (!r) return GFR_EINVAL;
@@ -516,7 +600,7 @@ int gfr_recon_profile(gfr_recon* r, int enable) {
 
 int gfr_recon_phase_times(gfr_recon* r, double* ms, long long* counts) {
   if (!r) return GFR_EINVAL;
-  GFR_HIP(r->ctx, hipStreamSynchronize(r->st));
+  GFR_HIP(r->ctx, sync_all(r));
   for (auto& pe : r->ev_pending) {
     float t = 0.f;
     if (hipEventElapsedTime(&t, pe.second.first, pe.second.second) == hipSuccess) {
@@ -554,6 +638,7 @@ int gfr_osem_run(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params* p,
 
 int gfr_recon_destroy(gfr_recon* r) {
   if (!r) return GFR_EINVAL;
+  if (r->stT) (void)hipStreamSynchronize(r->stT);
   if (r->st) (void)hipStreamSynchronize(r->st);
   if (r->vl) gfr_vlor_destroy(r->vl);
   for (auto& pe : r->ev_pending) {
@@ -562,6 +647,9 @@ int gfr_recon_destroy(gfr_recon* r) {
   }
   for (hipEvent_t e : r->ev_free) (void)hipEventDestroy(e);
   for (void* q : r->owned) (void)hipFree(q);
+  for (hipEvent_t e : {r->ev_snap, r->ev_tfree, r->ev_join})
+    if (e) (void)hipEventDestroy(e);
+  if (r->stT) (void)hipStreamDestroy(r->stT);
   if (r->st) (void)hipStreamDestroy(r->st);
   {
     std::lock_guard<std::mutex> lk(r->ctx->mu);

@@ -307,7 +307,12 @@ int gfr_recon_get(gfr_recon* r, int what, float* host_out);
  * (DRV:413-422 only sums prompts, projection and background). Passes that
  * fit the VLOR pseudo-sinogram (triples in mode 3) have none: GFR_ESTATE. */
 int gfr_recon_loglik(gfr_recon* r, double* out);
+/* The session's stream. In the joint modes the triples passes run on a
+ * second stream of the session (beside the next iteration's doubles
+ * passes) that joins this one when the last pass is enqueued. */
 void* gfr_recon_stream(gfr_recon* r);
+/* Waits for every pass enqueued so far (both streams of a joint session). */
+int gfr_recon_sync(gfr_recon* r);
 int gfr_recon_destroy(gfr_recon* r);
 
 /* Device-time tracing per phase of a pass (HIP events on the session's

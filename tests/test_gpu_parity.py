@@ -106,7 +106,7 @@ def test_odd_geometry_operators(gctx, orc):
         ne, de = orc.back(g, corr, np.where(valid, 1, -1000).astype(np.float32), s, S)
         for zb in (4, 5, 6, 8, 16):
             for kind, blk in ((0, 4), (0, 1)):
-                gctx.set_tuning(g, S, zb, 8, back_lds=kind, back_bw=blk)
+                set_back_shape(gctx, g, S, (zb, kind, 0, blk))
                 n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
                 what = "odd zb=%d kind=%d blk=%d S=%d" % (zb, kind, blk, S)
                 close(n, ne, rtol=0, what="numer " + what)
@@ -114,11 +114,16 @@ def test_odd_geometry_operators(gctx, orc):
     close(gctx.blur_xy(g, x, 1.5), orc.blur_xy(g, x, 1.5), rtol=0, what="blur odd")
 
 
-# every backprojector candidate of the autotuner: (slices, LDS window, register
-# budget, waves per block)
+# every backprojector candidate of the autotuner: (slices, kind: 0 cached
+# reads, 1 LDS window; register budget; waves per block)
 BACK_SHAPES = [(4, 0, 0, 4), (5, 0, 0, 4), (6, 0, 0, 4), (8, 0, 0, 4), (8, 0, 6, 4),
                (16, 0, 0, 4), (4, 0, 0, 1), (6, 0, 0, 1), (8, 0, 0, 1), (4, 1, 0, 4),
                (8, 1, 0, 4), (16, 1, 0, 4)]
+
+
+def set_back_shape(gctx, g, S, shape):
+    zb, kind, waves, blk = shape
+    gctx.set_tuning(g, S, zb, 8, back_lds=kind == 1, back_waves=waves, back_bw=blk)
 
 
 def test_reference_geometry_operators(gctx, orc):
@@ -148,10 +153,10 @@ def test_reference_geometry_operators(gctx, orc):
     valid = np.broadcast_to(inside, (g.nsinogs, g.nang, g.nrad)).copy().reshape(-1)
     valid[rng.integers(0, nd, size=nd // 50)] = False
     ne, de = orc.back(g, corr, np.where(valid, 1, -1000).astype(np.float32), s, S)
-    for zb, lds, bw, blk in BACK_SHAPES:
-        gctx.set_tuning(g, S, zb, 8, back_lds=lds, back_waves=bw, back_bw=blk)
+    for shape in BACK_SHAPES:
+        set_back_shape(gctx, g, S, shape)
         n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
-        what = "ref zb=%d lds=%d waves=%d blk=%d" % (zb, lds, bw, blk)
+        what = "ref zb=%d kind=%d waves=%d blk=%d" % shape
         close(n, ne, what="numer " + what)
         close(d, de, what="denom " + what)
     gctx.set_tuning(g, S, 8, 8)
@@ -217,10 +222,13 @@ def test_back_project_structured_mask(gctx, orc, g, s, S):
     dead = rng.integers(0, nd, size=max(8, nd // 20000))
     valid.reshape(-1)[dead] = False
     valid = valid.reshape(-1)
-    n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
     ne, de = orc.back(g, corr, np.where(valid, 1, -1000).astype(np.float32), s, S)
-    close(n, ne, rtol=0, what="numer")
-    close(d, de, rtol=0, what="denom")
+    for shape in ((6, 0, 0, 4), (8, 1, 0, 4), (8, 0, 0, 1), (4, 0, 0, 4)):
+        set_back_shape(gctx, g, S, shape)
+        n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
+        close(n, ne, rtol=0, what="numer zb=%d kind=%d waves=%d blk=%d" % shape)
+        close(d, de, rtol=0, what="denom zb=%d kind=%d waves=%d blk=%d" % shape)
+    gctx.set_tuning(g, S, 8, 8)
 
 
 def test_back_project_c1_subset_vs_oracle(gctx, orc):
@@ -234,11 +242,14 @@ def test_back_project_c1_subset_vs_oracle(gctx, orc):
     ne, de = orc.back(g, corr, cal, 5, 80)
     close(n, ne, what="numer c1")
     close(d, de, what="denom c1")
-    # the LDS-window backprojector on the same views (windows at full tilt)
-    gctx.set_tuning(g, 80, 8, 8, back_lds=1)
-    n2, d2 = gctx.back_project(g, corr, valid.astype(np.uint8), 5, 80)
-    close(n2, ne, rtol=0, what="numer c1 lds")
-    close(d2, de, rtol=0, what="denom c1 lds")
+    # the LDS-window backprojector (windows at full tilt) and the one-wave
+    # blocks of the cached-read kernel on the same views
+    for shape in ((8, 1, 0, 4), (6, 0, 0, 1)):
+        set_back_shape(gctx, g, 80, shape)
+        n2, d2 = gctx.back_project(g, corr, valid.astype(np.uint8), 5, 80)
+        close(n2, ne, rtol=0, what="numer c1 zb=%d kind=%d waves=%d blk=%d" % shape)
+        close(d2, de, rtol=0, what="denom c1 zb=%d kind=%d waves=%d blk=%d" % shape)
+    gctx.set_tuning(g, 80, 8, 8)
 
 
 # back kind: 0 cached reads, 1 per-block LDS row window; bw: register budget
@@ -408,11 +419,14 @@ def make_study(g, seed, mode, counts=2e5):
 
 # c1-m1-S16: one whole iteration (16 subset passes) at the Inveon size, about
 # 15 s for the oracle on 16 host threads
+# The joint cases with 2-3 iterations cross iteration boundaries, where the
+# triples passes of one iteration run beside the next iteration's doubles.
 @pytest.mark.parametrize("g,mode,S,niter,counts", [(C0, 1, 4, 2, 2e5), (SMALL, 2, 3, 2, 2e5),
                                                     (SMALL, 3, 1, 2, 2e5), (SMALL, 3, 4, 1, 2e5),
-                                                    (ODD, 2, 5, 2, 1e5), (C1, 1, 16, 1, 5e7)],
+                                                    (ODD, 2, 5, 2, 1e5), (C1, 1, 16, 1, 5e7),
+                                                    (C0, 3, 4, 3, 3e5), (C0, 2, 8, 2, 3e5)],
                          ids=["c0-m1-S4", "small-m2-S3", "small-m3-S1", "small-m3-S4",
-                              "odd-m2-S5", "c1-m1-S16"])
+                              "odd-m2-S5", "c1-m1-S16", "c0-m3-S4-3it", "c0-m2-S8-2it"])
 def test_recon_vs_oracle(gctx, orc, g, mode, S, niter, counts):
     d2, t2, norm, rec = make_study(g, 12 + mode, mode, counts)
     p = ga.recon_params(mode=mode, niter=niter, nsub=S)
