@@ -622,7 +622,7 @@ def run_c3(args):
 
     def make_batch(lanes, sub):
         bp = ga.batch_params(storage=storage, max_frames=len(sub), min_nsub=min(f["nsub"] for f in sub),
-                             lanes=lanes)
+                             lanes=lanes, group=args.group)
         bt = ctx.batch(g, bp, norm)
         c3_fill(ctx, bt, sub, P, tot, niter, ybuf, args.seed)
         return bt
@@ -701,7 +701,8 @@ def run_c3(args):
         "config": {"workload": "c3 batched dynamic study: %d frames of the Inveon-scale "
                                "geometry, subsets {8,16,24}, %d iterations, counts "
                                "log-U[1e5,1e7]" % (len(frames), niter),
-                   "frames": len(frames), "lanes": lanes, "storage": sname},
+                   "frames": len(frames), "lanes": lanes, "storage": sname,
+                   "group": args.group or 3},
         "passes": passes,
         "passes_per_s": round(passes / dt, 2),
         "seconds": round(dt, 3),
@@ -1032,6 +1033,8 @@ def main():
     ap.add_argument("--frames", type=int, default=64)
     ap.add_argument("--c3-niter", type=int, default=4)
     ap.add_argument("--lanes", type=int, default=0, help="0: measure 1..4")
+    ap.add_argument("--group", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="c3: frames per shared backprojection (1: none; 0: 3)")
     ap.add_argument("--plain-frames", type=int, default=6)
     ap.add_argument("--error-report", action="store_true")
     ap.add_argument("--seed", type=int, default=3)

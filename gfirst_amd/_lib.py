@@ -48,7 +48,8 @@ class GfrTuning(C.Structure):
 
 class GfrBatchParams(C.Structure):
     _fields_ = [("storage", C.c_int), ("max_frames", C.c_int), ("min_nsub", C.c_int),
-                ("lanes", C.c_int), ("psf_fw", C.c_float), ("psf_bw", C.c_float)]
+                ("lanes", C.c_int), ("psf_fw", C.c_float), ("psf_bw", C.c_float),
+                ("group", C.c_int)]
 
 
 class GfrRequestInfo(C.Structure):

@@ -423,8 +423,10 @@ class Recon:
 
 
 def batch_params(storage=_lib.STORE_F32, max_frames=64, min_nsub=8, lanes=2, psf_fw=1.5,
-                 psf_bw=1.5):
-    return GfrBatchParams(storage, max_frames, min_nsub, lanes, psf_fw, psf_bw)
+                 psf_bw=1.5, group=0):
+    """group: frames at the same subset of the same subset count that share one
+    backprojection launch (1 none, 2 or 3; 0 = 3); results do not depend on it."""
+    return GfrBatchParams(storage, max_frames, min_nsub, lanes, psf_fw, psf_bw, group)
 
 
 def batch_footprint(g, params, frames=0, has_bg=False):

@@ -13,7 +13,12 @@
 // of frame f is sub[f]) and a final kernel advances every index, so the same
 // graph serves every step until a frame joins or finishes. The pass itself is
 // the reference's (GFIRST.cu:324-340 blur, 341-469 forward/ratio/backward,
-// 482-502 update), identical kernel for kernel to gfr_recon_step in mode 1.
+// 482-502 update), identical kernel for kernel to gfr_recon_step in mode 1,
+// except that frames at the same subset of the same subset count share their
+// backprojection: up to `group` of them write their ratios into one set of
+// interleaved rows and one launch backprojects them all (the per-view
+// geometry, calibration interpolation and row loads are shared; each frame's
+// sums keep their bits).
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -32,6 +37,9 @@ struct Lane {
   hipEvent_t join = nullptr;
   float *xb = nullptr, *num = nullptr, *den = nullptr, *work = nullptr, *R = nullptr;
   float2* corr2 = nullptr;
+  float* corr4 = nullptr;                // interleaved rows of a frame group
+  float* num_g[kMaxGroup] = {};          // numerators / denominators of a group
+  float* den_g[kMaxGroup] = {};          // ([0] = num / den)
   double* llh_part = nullptr;
   size_t r_floats = 0;
 };
@@ -50,8 +58,11 @@ struct Frame {
 
 size_t store_bytes(int storage) { return storage == GFR_STORE_F32 ? 4 : 2; }
 
+// Frames per shared backprojection (0 asks for the largest).
+int group_of(const gfr_batch_params* p) { return p->group == 0 ? kMaxGroup : p->group; }
+
 struct Sizes {
-  size_t img, corr2, rot, llh, frame_nobg, frame_bg, shared;
+  size_t img, corr2, corr4, rot, llh, frame_nobg, frame_bg, shared;
 };
 
 Sizes sizes_of(const GeomHost& h, const gfr_batch_params* p) {
@@ -60,13 +71,16 @@ Sizes sizes_of(const GeomHost& h, const gfr_batch_params* p) {
   z.img = sizeof(float) * (size_t)h.info.nvox;
   z.corr2 = sizeof(float2) * nav * ((size_t)h.info.nsinogs + 2 * (size_t)h.g.nseg) *
             ((size_t)h.g.nrad + 2);  // packed rows: NRAD + 2 entries
+  const int grp = group_of(p);
+  z.corr4 = grp > 1 ? 2 * z.corr2 : 0;
   z.rot = sizeof(float) * (size_t)(h.info.lambda1 - h.info.lambda0 + 1) * h.g.nzs * h.g.nrad *
           std::min(nav, (size_t)kRotChunk);
   z.llh = sizeof(double) * (size_t)llh_capacity(h, p->min_nsub) + 64;
   const size_t sino = store_bytes(p->storage) * (size_t)h.info.ndata;
   z.frame_nobg = z.img + sino;
   z.frame_bg = z.img + 2 * sino;
-  const size_t per_lane = 4 * z.img + z.corr2 + z.rot + z.llh;
+  const size_t per_lane =
+      (4 + 2 * (size_t)(grp - 1)) * z.img + z.corr2 + z.corr4 + z.rot + z.llh;
   const size_t slots = (size_t)p->max_frames * (3 * sizeof(int) + sizeof(double)) + 256;
   z.shared = (size_t)p->lanes * per_lane + (size_t)h.info.ndata /* mask */ + slots;
   return z;
@@ -112,20 +126,23 @@ void drop_graph(gfr_batch* b) {
   b->graph = nullptr;
 }
 
-// Enqueues one subset pass of frame f on lane L (kernels read the subset
-// index from d_sub[f]).
-hipError_t enqueue_pass(gfr_batch* b, int f, const Lane& L) {
+// Enqueues the blur and forward projection of frame f on lane L (kernels
+// read the subset index from d_sub[f]); the ratio goes to the lane's packed
+// rows (chan < 0) or to channel chan of its interleaved group rows.
+hipError_t enqueue_forward(gfr_batch* b, int f, const Lane& L, int chan) {
   const DevGeom& G = b->gd->dev;
   const Frame& fr = b->frames[f];
   const int* dsub = b->d_sub + f;
-  Taps9 tf, tb;
-  Taps5 tz;
+  Taps9 tf;
   psf_taps(b->p.psf_fw, tf.g);
-  psf_taps(b->p.psf_bw, tb.g);
-  gz_taps(tz.g);
   hipError_t e = launch_blur_xy(G, tf, fr.x, L.xb, L.st);
   FwdOut fo;
-  fo.corr2 = L.corr2;
+  if (chan < 0) {
+    fo.corr2 = L.corr2;
+  } else {
+    fo.corr4 = L.corr4;
+    fo.chan = chan;
+  }
   fo.mask = b->mask;
   fo.y = fr.y;
   fo.bg = fr.bg;
@@ -136,38 +153,94 @@ hipError_t enqueue_pass(gfr_batch* b, int f, const Lane& L) {
     e = launch_forward(G, L.xb, L.R, L.r_floats, fo, 0, fr.nsub, dsub, fr.tune, L.st);
   if (e == hipSuccess)
     e = launch_llh_sum(G, L.llh_part, 0, fr.nsub, dsub, fr.tune.zsum_sg, b->d_llh + f, L.st);
-  if (e == hipSuccess)
-    e = launch_back_packed(G, L.corr2, L.num, L.den, 0, fr.nsub, dsub, fr.tune, L.st);
-  if (e == hipSuccess) e = launch_update(G, tb, tz, L.num, L.den, fr.x, L.work, L.st);
   return e;
 }
 
-// Work of one pass of a frame, for spreading frames over lanes.
-double pass_cost(const gfr_batch* b, int f) {
-  const int S = b->frames[f].nsub;
-  return (double)((b->g.nang + S - 1) / S) + 2.0;
+hipError_t enqueue_update(gfr_batch* b, int f, const float* num, const float* den,
+                          const Lane& L) {
+  Taps9 tb;
+  Taps5 tz;
+  psf_taps(b->p.psf_bw, tb.g);
+  gz_taps(tz.g);
+  return launch_update(b->gd->dev, tb, tz, num, den, b->frames[f].x, L.work, L.st);
+}
+
+// Enqueues one subset pass of the frames of a group on lane L: every
+// member's forward projection, one backprojection, every member's update.
+hipError_t enqueue_group(gfr_batch* b, const std::vector<int>& grp, const Lane& L) {
+  const DevGeom& G = b->gd->dev;
+  const Frame& f0 = b->frames[grp[0]];
+  hipError_t e = hipSuccess;
+  if (grp.size() == 1) {
+    e = enqueue_forward(b, grp[0], L, -1);
+    if (e == hipSuccess)
+      e = launch_back_packed(G, L.corr2, L.num, L.den, 0, f0.nsub, b->d_sub + grp[0], f0.tune,
+                             L.st);
+    if (e == hipSuccess) e = enqueue_update(b, grp[0], L.num, L.den, L);
+    return e;
+  }
+  for (size_t i = 0; i < grp.size() && e == hipSuccess; i++)
+    e = enqueue_forward(b, grp[i], L, (int)i);
+  if (e == hipSuccess)
+    e = launch_back_multi(G, L.corr4, L.num_g, L.den_g, (int)grp.size(), 0, f0.nsub,
+                          b->d_sub + grp[0], f0.tune.multi_zb == 6 ? 6 : 4, L.st);
+  for (size_t i = 0; i < grp.size() && e == hipSuccess; i++)
+    e = enqueue_update(b, grp[i], L.num_g[i], L.den_g[i], L);
+  return e;
+}
+
+// Work of one pass of a group, for spreading groups over lanes (a shared
+// backprojection costs about 0.7 of the separate ones per frame).
+double group_cost(const gfr_batch* b, const std::vector<int>& grp) {
+  const int S = b->frames[grp[0]].nsub;
+  const double one = (double)((b->g.nang + S - 1) / S) + 2.0;
+  return grp.size() == 1 ? one : 0.7 * one * (double)grp.size();
+}
+
+// Frames at the same subset of the same subset count (their subset indices
+// stay equal while the membership is unchanged: every active frame advances
+// once per step) form groups of at most `group`.
+std::vector<std::vector<int>> make_groups(const gfr_batch* b) {
+  std::vector<int> order = b->active;
+  auto key = [&](int f) {
+    const Frame& fr = b->frames[f];
+    return std::make_pair(fr.nsub, (int)(fr.passes % fr.nsub));
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return key(a) < key(c); });
+  const int gmax = group_of(&b->p);
+  std::vector<std::vector<int>> groups;
+  for (int f : order) {
+    if (!groups.empty() && (int)groups.back().size() < gmax && key(groups.back()[0]) == key(f))
+      groups.back().push_back(f);
+    else
+      groups.push_back({f});
+  }
+  return groups;
 }
 
 // Enqueues one step (every active frame, then the index advance) on the
 // origin stream; under capture this records the step program.
 hipError_t enqueue_step(gfr_batch* b) {
   const int nl = (int)b->lanes.size();
-  std::vector<int> order = b->active;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int c) { return pass_cost(b, a) > pass_cost(b, c); });
+  std::vector<std::vector<int>> groups = make_groups(b);
+  std::stable_sort(groups.begin(), groups.end(), [&](const std::vector<int>& a,
+                                                     const std::vector<int>& c) {
+    return group_cost(b, a) > group_cost(b, c);
+  });
   std::vector<double> load(nl, 0.0);
-  std::vector<std::vector<int>> per(nl);
-  for (int f : order) {
+  std::vector<std::vector<const std::vector<int>*>> per(nl);
+  for (const auto& grp : groups) {
     const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    load[l] += pass_cost(b, f);
-    per[l].push_back(f);
+    load[l] += group_cost(b, grp);
+    per[l].push_back(&grp);
   }
   hipError_t e = hipEventRecord(b->fork, b->st);
   for (int l = 0; l < nl && e == hipSuccess; l++) {
     if (per[l].empty()) continue;
     Lane& L = b->lanes[l];
     e = hipStreamWaitEvent(L.st, b->fork, 0);
-    for (size_t i = 0; i < per[l].size() && e == hipSuccess; i++) e = enqueue_pass(b, per[l][i], L);
+    for (size_t i = 0; i < per[l].size() && e == hipSuccess; i++)
+      e = enqueue_group(b, *per[l][i], L);
     if (e == hipSuccess) e = hipEventRecord(L.join, L.st);
     if (e == hipSuccess) e = hipStreamWaitEvent(b->st, L.join, 0);
   }
@@ -205,6 +278,29 @@ int rebuild(gfr_batch* b) {
   return GFR_OK;
 }
 
+// Best of two timed group backprojections (after a warm-up) of subset 0.
+float time_group_back(gfr_batch* b, Lane& L, int nsub, int zb, hipError_t* err) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  float best = 1e30f;
+  for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+    e = hipEventRecord(e0, b->st);
+    if (e == hipSuccess)
+      e = launch_back_multi(b->gd->dev, L.corr4, L.num_g, L.den_g, group_of(&b->p), 0, nsub,
+                            nullptr, zb, b->st);
+    if (e == hipSuccess) e = hipEventRecord(e1, b->st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && rep > 0 && ms < best) best = ms;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  *err = e;
+  return best;
+}
+
 bool any_positive(const float* v, long long n) {
   for (long long i = 0; i < n; i++)
     if (!(v[i] <= 0.f)) return true;
@@ -217,8 +313,8 @@ extern "C" {
 
 long long gfr_batch_footprint(const gfr_geometry* g, const gfr_batch_params* p) {
   GeomHost h;
-  if (!p || p->lanes < 1 || p->min_nsub < 1 || p->max_frames < 1 ||
-      derive_geometry(g, &h, nullptr) != GFR_OK)
+  if (!p || p->lanes < 1 || p->min_nsub < 1 || p->max_frames < 1 || p->group < 0 ||
+      p->group > kMaxGroup || derive_geometry(g, &h, nullptr) != GFR_OK)
     return GFR_EINVAL;
   return (long long)sizes_of(h, p).shared;
 }
@@ -244,6 +340,8 @@ int gfr_batch_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_batch_params
   if (p->lanes < 1 || p->lanes > 16) return set_err(ctx, GFR_EINVAL, "lanes must be in [1, 16]");
   if (p->min_nsub < 1 || p->min_nsub > g->nang)
     return set_err(ctx, GFR_EINVAL, "min_nsub must be in [1, nang]");
+  if (p->group < 0 || p->group > kMaxGroup)
+    return set_err(ctx, GFR_EINVAL, "group must be in [0, 3]");
   if (g->nzs < 3) return set_err(ctx, GFR_EINVAL, "nzs must be >= 3");
   const GeomHost& h = gd->host;
   const Sizes z = sizes_of(h, p);
@@ -281,6 +379,15 @@ int gfr_batch_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_batch_params
          A((void**)&L.xb, z.img) && A((void**)&L.num, z.img) && A((void**)&L.den, z.img) &&
          A((void**)&L.work, z.img) && A((void**)&L.corr2, z.corr2) && A((void**)&L.R, z.rot) &&
          A((void**)&L.llh_part, z.llh);
+    L.num_g[0] = L.num;
+    L.den_g[0] = L.den;
+    if (ok && z.corr4) {
+      ok = A((void**)&L.corr4, z.corr4);
+      for (int i = 1; i < group_of(p) && ok; i++)
+        ok = A((void**)&L.num_g[i], z.img) && A((void**)&L.den_g[i], z.img);
+      // defined contents for the first timing of the group backprojector
+      if (ok) ok = launch_zero(L.corr4, z.corr4, b->st) == hipSuccess;
+    }
     L.r_floats = z.rot / sizeof(float);
   }
   if (!ok) return bfail(b, set_err(ctx, GFR_ENOMEM, "batch allocation failed"));
@@ -407,6 +514,21 @@ int gfr_batch_add(gfr_batch* b, int niter, int nsub, const float* prompts, const
     if (e != hipSuccess) return undo(hip_err(ctx, e, "gfr_batch_add tuning"));
     std::lock_guard<std::mutex> lk(ctx->mu);
     b->gd->tune[nsub] = t;
+  }
+  // Slices of the group backprojector, measured once per subset count.
+  if (t.multi_zb == 0 && group_of(&b->p) > 1 && !(env && env[0] == '0')) {
+    Lane& L = b->lanes[0];
+    float best = 1e30f;
+    for (int zb : {4, 6}) {
+      const float ms = time_group_back(b, L, nsub, zb, &e);
+      if (e != hipSuccess) return undo(hip_err(ctx, e, "gfr_batch_add tuning"));
+      if (ms < best) {
+        best = ms;
+        t.multi_zb = zb;
+      }
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    b->gd->tune[nsub].multi_zb = t.multi_zb;
   }
   (void)hipFree(stage);
   stage = nullptr;

@@ -103,4 +103,36 @@ __device__ __forceinline__ void store_packed(const DevGeom& G, float2* c2, int a
   if (k == nz - 1) put(row0 + nz + 1, edge);
 }
 
+// Interleaved rows of up to three frames of one geometry, calibration mask
+// and subset: the layout above with 16-byte entries {ratio_0, ratio_1,
+// ratio_2, calib}. Writes frame ch's ratio and the calibration (the same
+// value from every frame) of one bin, with the same halo entries and extra
+// rows.
+__device__ __forceinline__ void store_packed4(const DevGeom& G, float* c4, int ag, int s, int r,
+                                              int ch, float ratio, float calib) {
+  const int2 sp = G.sino[s];
+  const SegEntry se = G.seg[sp.x];
+  const size_t row0 = (size_t)ag * packed_rows_per_angle(G) + se.nza + 2 * (size_t)sp.x;
+  const int k = sp.y - se.zmin;
+  const int nz = G.nzs - 2 * se.zmin;
+  const bool direct = se.zmin == 0;
+  const size_t L = packed_row_len(G.nrad);
+  auto put = [&](size_t row, float rv, float cv) {
+    float* p = c4 + (row * L + r + 1) * 4;
+    p[ch] = rv;
+    p[3] = cv;
+    if (r == 0) {
+      p[ch - 4] = rv;
+      p[-1] = cv;
+    }
+    if (r == G.nrad - 1) {
+      p[4 + ch] = rv;
+      p[7] = cv;
+    }
+  };
+  put(row0 + 1 + k, ratio, calib);
+  if (k == 0) put(row0, direct ? ratio : -1000.f, direct ? calib : -1000.f);
+  if (k == nz - 1) put(row0 + nz + 1, direct ? ratio : -1000.f, direct ? calib : -1000.f);
+}
+
 }  // namespace gfr

@@ -52,6 +52,7 @@ struct Tune {
   int back_waves = 0;     // register budget for this many waves per SIMD (0: compiler's)
   int rot_z = 16;         // planes per rotation block: 4, 16 or 32
   int back_bw = 4;        // waves per direct backprojector block: 4 (16x16) or 1 (8x8)
+  int multi_zb = 0;       // slices of the multi-frame backprojector: 4 or 6 (0: not measured)
   bool tuned = false;
   float ms_back = 0.f, ms_zsum = 0.f;
 };
@@ -165,6 +166,8 @@ struct FwdOut {
   float* proj = nullptr;          // projection, full sinogram layout
   float* corr = nullptr;          // ratio, full layout
   float2* corr2 = nullptr;        // {ratio, calib} packed [angle of subset][s][r]
+  float* corr4 = nullptr;         // or: channel `chan` of interleaved 3-frame rows
+  int chan = 0;
   const uint8_t* mask = nullptr;  // calibration mask (needed with corr2)
   const void* y = nullptr;        // measured prompts (storage format, unscaled)
   const void* bg = nullptr;       // background or NULL
@@ -185,6 +188,14 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
 hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* numer, float* denom,
                               int subset, int nsub, const int* dsub, const Tune& tune,
                               hipStream_t st);
+// Multi-frame backprojection: nf (2 or 3) frames' interleaved rows
+// (packed4_bytes) -> numer[f], denom[f]; the sums of launch_back_packed on
+// each frame's own rows, bit for bit. zb: 4 or 6 slices per thread.
+constexpr int kMaxGroup = 3;
+size_t packed4_bytes(const DevGeom& G, int subset, int nsub);
+hipError_t launch_back_multi(const DevGeom& G, const float* corr4, float* const* numer,
+                             float* const* denom, int nf, int subset, int nsub, const int* dsub,
+                             int zb, hipStream_t st);
 // Sensitivity of a subset: the backprojector's denominator half in FP64
 // (packed rows carry ratio 1, so only calibration gaps apply).
 hipError_t launch_sensitivity(const DevGeom& G, const float2* corr2, double* sens, int subset,

@@ -161,7 +161,9 @@ struct FwdArgs {
   float* proj;           // full sinogram layout, or NULL
   float* corr;           // full layout ratio, or NULL
   float2* corr2;         // packed [angle of subset][s][r] {ratio, calib}, or NULL
-  const uint8_t* mask;   // calibration mask (full layout) for corr2
+  float* corr4;          // interleaved rows of a frame group (channel chan), or NULL
+  int chan;
+  const uint8_t* mask;   // calibration mask (full layout) for corr2 / corr4
   const void* y;         // measured prompts (storage format), unscaled
   const void* bg;        // background or NULL
   float yscale;          // RES (GFIRST.cu:224-260)
@@ -281,6 +283,8 @@ __global__ __launch_bounds__(256) void k_zsum(DevGeom G, const float* __restrict
             if (o.corr) o.corr[bin] = c;
             if (o.corr2)
               store_packed(G, o.corr2, ag, s, rr2, make_float2(c, o.mask[bin] ? 1.f : -1000.f));
+            if (o.corr4)
+              store_packed4(G, o.corr4, ag, s, rr2, o.chan, c, o.mask[bin] ? 1.f : -1000.f);
             if constexpr (LLH) {
               // Poisson log-likelihood in count units over the bins whose
               // ratio enters the backprojection: model mean den/RES, counts
@@ -749,6 +753,194 @@ __global__ __launch_bounds__(64 * BW, MINW > 0 ? MINW : 1) void k_back_direct(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Multi-frame backprojector: F frames (2 or 3) of one geometry, calibration
+// mask and subset in one launch. Their packed rows are interleaved per entry
+// as {ratio_0, ratio_1, ratio_2, calib} (store_packed4), so one 16-byte load
+// gives a bin of every frame, and each lane's view geometry (rotation, radial
+// and plane positions, the shared-z test, the skip ballot) and the
+// calibration interpolation are computed once for the group. Per frame the
+// sums are those k_back_direct forms from the frame's own {ratio, calib}
+// rows: the same interpolants from the same values, the same gap rule (a
+// negative interpolated ratio or calibration drops the view for that frame
+// only) and the same view order, so the bits are identical.
+// ---------------------------------------------------------------------------
+struct FetchGlobal4 {
+  __amdgpu_buffer_rsrc_t rc;
+  unsigned vrow, rowb, oq, rb0 = 0;
+  __device__ __forceinline__ void rows(int i0) { rb0 = (vrow + (unsigned)i0) * rowb; }
+  __device__ __forceinline__ void pair(int k, float4& a0, float4& a1) const {
+    a0 = ld_f4(rc, rb0 + oq, (unsigned)k * rowb);
+    a1 = ld_f4(rc, rb0 + oq + 16u, (unsigned)k * rowb);
+  }
+  __device__ __forceinline__ void get2(int ii, float4& a0, float4& a1) const {
+    const unsigned o = (vrow + (unsigned)ii) * rowb + oq;
+    a0 = ld_f4(rc, o, 0);
+    a1 = ld_f4(rc, o + 16u, 0);
+  }
+};
+
+__device__ __forceinline__ float chan_of(const float4& v, int f) {
+  return f == 0 ? v.x : (f == 1 ? v.y : v.z);
+}
+
+template <int ZB, int F>
+__device__ __forceinline__ void accumulate_view_multi(const LaneView& v, float fzb, float yr,
+                                                      int zb, int nzs, float fzm, float fzhi,
+                                                      FetchGlobal4& fetch, float (&n)[F][ZB],
+                                                      float (&d)[F][ZB]) {
+  // shared z axis: the argument of accumulate_view
+  const bool slow = v.any && !(v.s0 >= 1.f && (__float_as_uint(v.s0) >> 23) ==
+                                                   (__float_as_uint(v.wl) >> 23));
+  if (__builtin_amdgcn_ballot_w64(slow) == 0) {
+    const float f0 = floorf(v.s0);
+    const float al = v.s0 - f0;
+    const int i0 = (int)f0;
+    float c[F][ZB + 1], q[ZB + 1];
+    fetch.rows(i0);
+#pragma unroll
+    for (int k = 0; k <= ZB; k++) {
+      float4 a0, a1;
+      fetch.pair(k, a0, a1);
+      q[k] = lerp1(a0.w, a1.w, v.au);
+#pragma unroll
+      for (int f = 0; f < F; f++) c[f][k] = lerp1(chan_of(a0, f), chan_of(a1, f), v.au);
+    }
+#pragma unroll
+    for (int j = 0; j < ZB; j++) {
+      const float wj = v.w0 + (float)j;
+      const float B = lerp1(q[j], q[j + 1], al);
+      const bool base = v.any && (zb + j < nzs) && wj >= fzm && wj < fzhi && !(B < 0.f);
+#pragma unroll
+      for (int f = 0; f < F; f++) {
+        const float A = lerp1(c[f][j], c[f][j + 1], al);
+        const bool ok = base && !(A < 0.f);
+        n[f][j] += ok ? A : 0.f;
+        d[f][j] += ok ? B : 0.f;
+      }
+    }
+    return;
+  }
+  float al[ZB];
+  int ii[ZB];
+  bool inr[ZB];
+#pragma unroll
+  for (int j = 0; j < ZB; j++) {
+    const float w = (j == 0) ? v.w0 : (j == ZB - 1) ? v.wl : fmaf(yr, fzb, (float)(zb + j)) + 0.5f;
+    inr[j] = v.any && (zb + j < nzs) && w >= fzm && w < fzhi;
+    const float t = w - 0.5f;
+    const float fl = floorf(t);
+    al[j] = t - fl;
+    ii[j] = (int)fl;
+  }
+  bool reg = true;
+#pragma unroll
+  for (int j = 1; j < ZB; j++) reg = reg && (!inr[j] || ii[j] == ii[0] + j);
+  float av[F][ZB], bv[ZB];
+  if (reg) {
+    float c[F][ZB + 1], q[ZB + 1];
+    fetch.rows(ii[0]);
+#pragma unroll
+    for (int k = 0; k <= ZB; k++) {
+      float4 a0, a1;
+      fetch.pair(k, a0, a1);
+      q[k] = lerp1(a0.w, a1.w, v.au);
+#pragma unroll
+      for (int f = 0; f < F; f++) c[f][k] = lerp1(chan_of(a0, f), chan_of(a1, f), v.au);
+    }
+#pragma unroll
+    for (int j = 0; j < ZB; j++) {
+      bv[j] = inr[j] ? lerp1(q[j], q[j + 1], al[j]) : 0.f;
+#pragma unroll
+      for (int f = 0; f < F; f++) av[f][j] = inr[j] ? lerp1(c[f][j], c[f][j + 1], al[j]) : 0.f;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < ZB; j++) {
+      bv[j] = 0.f;
+#pragma unroll
+      for (int f = 0; f < F; f++) av[f][j] = 0.f;
+      if (inr[j]) {
+        float4 w00, w01, w10, w11;
+        fetch.get2(ii[j], w00, w01);
+        fetch.get2(ii[j] + 1, w10, w11);
+        bv[j] = lerp1(lerp1(w00.w, w01.w, v.au), lerp1(w10.w, w11.w, v.au), al[j]);
+#pragma unroll
+        for (int f = 0; f < F; f++)
+          av[f][j] = lerp1(lerp1(chan_of(w00, f), chan_of(w01, f), v.au),
+                           lerp1(chan_of(w10, f), chan_of(w11, f), v.au), al[j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < ZB; j++) {
+#pragma unroll
+    for (int f = 0; f < F; f++) {
+      const bool gap = bv[j] < 0.f || av[f][j] < 0.f;  // GFIRST_GPU_Kernels.cu:157
+      n[f][j] += gap ? 0.f : av[f][j];
+      d[f][j] += gap ? 0.f : bv[j];
+    }
+  }
+}
+
+struct MultiOut {
+  float* numer[kMaxGroup];
+  float* denom[kMaxGroup];
+};
+
+template <int ZB, int F>
+__global__ __launch_bounds__(256) void k_back_multi(DevGeom G, const float* __restrict__ c4,
+                                                    unsigned c4_bytes, MultiOut o, int subset,
+                                                    int nsub, const int* __restrict__ dsub) {
+  if (dsub) subset = *dsub;
+  const int nav = (G.nang - subset + nsub - 1) / nsub;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ix = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
+  const int iy = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
+  const int zb = blockIdx.z * ZB;
+  const bool act = ix < G.res && iy < G.res;
+  const __amdgpu_buffer_rsrc_t rc4 = rsrc(c4, c4_bytes);
+  const float xv = (float)ix - G.RESM, yv = (float)iy - G.RESM;
+  const unsigned rowb = (unsigned)packed_row_len(G.nrad) * 16u;
+  const unsigned nr = (unsigned)(G.nsinogs + 2 * G.nseg);
+  const float fz0 = (float)zb, fzl = (float)(zb + ZB - 1);
+  float n[F][ZB], d[F][ZB];
+#pragma unroll
+  for (int f = 0; f < F; f++)
+#pragma unroll
+    for (int j = 0; j < ZB; j++) n[f][j] = d[f][j] = 0.f;
+  for (int si = 0; si < G.nseg; si++) {
+    const SegEntry se = G.seg[si];
+    const float fzm = (float)se.zmin, fzhi = (float)(G.nzs - se.zmin);
+    const unsigned segrow = (unsigned)(se.nza + 2 * si + 1 - se.zmin);
+    for (int a = 0; a < nav; a++) {
+      const int it = subset + a * nsub;
+      const LaneView v = lane_view(G, xv, yv, G.sin_t[it], G.cos_t[it], se.fzb, fz0, fzl, fzm,
+                                   fzhi, act);
+      if (__builtin_amdgcn_ballot_w64(v.any) == 0) continue;
+      FetchGlobal4 fe;
+      fe.rc = rc4;
+      fe.vrow = (unsigned)a * nr + segrow;
+      fe.rowb = rowb;
+      fe.oq = (unsigned)(v.iq + 1) * 16u;
+      accumulate_view_multi<ZB, F>(v, se.fzb, v.yr, zb, G.nzs, fzm, fzhi, fe, n, d);
+    }
+  }
+  if (!act) return;
+#pragma unroll
+  for (int j = 0; j < ZB; j++) {
+    const int iz = zb + j;
+    if (iz < G.nzs) {
+      const size_t iV = ((size_t)iz * G.res + iy) * G.res + ix;
+#pragma unroll
+      for (int f = 0; f < F; f++) {
+        o.numer[f][iV] = n[f][j];
+        o.denom[f][iV] = d[f][j];
+      }
+    }
+  }
+}
+
 inline int nav_of(const DevGeom& G, int subset, int nsub) {
   return (G.nang - subset + nsub - 1) / nsub;
 }
@@ -836,8 +1028,10 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
   if (chunk < 1) return hipErrorInvalidValue;
   // log-likelihood partials are indexed by pair: chunks must hold whole pairs
   if (out.llh_part && chunk < nav && (chunk & 1)) return hipErrorInvalidValue;
-  FwdArgs o{out.proj, out.corr, out.corr2, out.mask, out.y, out.bg, out.yscale, out.llh_part};
-  const bool ratio = out.corr || out.corr2;
+  if (out.corr4 && (out.chan < 0 || out.chan >= kMaxGroup)) return hipErrorInvalidValue;
+  FwdArgs o{out.proj, out.corr,  out.corr2,  out.corr4,   out.chan,
+            out.mask, out.y,     out.bg,     out.yscale,  out.llh_part};
+  const bool ratio = out.corr || out.corr2 || out.corr4;
   for (int a0 = 0; a0 < nav; a0 += chunk) {
     const int na = min(chunk, nav - a0);
     const unsigned rbytes = (unsigned)((size_t)na * per * sizeof(float));
@@ -882,6 +1076,34 @@ int back_window_rows(const DevGeom& G, int zb) {
 size_t packed_bytes(const DevGeom& G, int subset, int nsub) {
   return sizeof(float2) * (size_t)nav_of(G, subset, nsub) * (G.nsinogs + 2 * (size_t)G.nseg) *
          packed_row_len(G.nrad);
+}
+
+size_t packed4_bytes(const DevGeom& G, int subset, int nsub) {
+  return 2 * packed_bytes(G, subset, nsub);
+}
+
+hipError_t launch_back_multi(const DevGeom& G, const float* corr4, float* const* numer,
+                             float* const* denom, int nf, int subset, int nsub, const int* dsub,
+                             int zb, hipStream_t st) {
+  const size_t bytes = packed4_bytes(G, dsub ? 0 : subset, nsub);
+  if (bytes >= 0xFFFFFFF0u || nf < 2 || nf > kMaxGroup) return hipErrorInvalidValue;
+  MultiOut o{};
+  for (int f = 0; f < nf; f++) {
+    if (!numer[f] || !denom[f]) return hipErrorInvalidValue;
+    o.numer[f] = numer[f];
+    o.denom[f] = denom[f];
+  }
+  const unsigned nbx = (G.res + 15) / 16;
+#define GFR_BM(ZB, F)                                                                          \
+  hipLaunchKernelGGL((k_back_multi<ZB, F>), dim3(nbx, nbx, (G.nzs + ZB - 1) / ZB), dim3(256), \
+                     0, st, G, corr4, (unsigned)bytes, o, subset, nsub, dsub)
+  if (zb == 6) {
+    if (nf == 3) GFR_BM(6, 3); else GFR_BM(6, 2);
+  } else {
+    if (nf == 3) GFR_BM(4, 3); else GFR_BM(4, 2);
+  }
+#undef GFR_BM
+  return hipGetLastError();
 }
 
 hipError_t launch_pack(const DevGeom& G, const float* corr, const uint8_t* mask, float2* c2,

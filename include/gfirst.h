@@ -347,6 +347,9 @@ typedef struct gfr_batch_params {
   int min_nsub;    /* smallest subset count a frame may use (sizes the scratch) */
   int lanes;       /* independent scratch chains, >= 1 */
   float psf_fw, psf_bw;
+  int group;       /* frames at the same subset of the same subset count that
+                      share one backprojection launch: 1 (none), 2 or 3; 0 = 3.
+                      Results do not depend on it. */
 } gfr_batch_params;
 /* Device bytes of the batch without frames, and of one frame. */
 long long gfr_batch_footprint(const gfr_geometry* g, const gfr_batch_params* p);

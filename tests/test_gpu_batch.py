@@ -89,6 +89,43 @@ def test_batch_frames_equal_sessions(gctx, storage, graph):
     bt.close()
 
 
+@pytest.mark.parametrize("storage,graph", [(ga.STORE_F32, True), (ga.STORE_F16, False),
+                                           (ga.STORE_BF16, True)],
+                         ids=["f32-graph", "f16-eager", "bf16-graph"])
+def test_frame_groups_equal_sessions(gctx, storage, graph):
+    """Frames at the same subset of the same subset count share one
+    backprojection launch (groups of up to three over interleaved rows). Seven
+    S=4 frames (groups 3+3+1, regrouped as members finish), two S=2 frames (a
+    group of two), one S=3 frame alone, and an S=4 frame that joins two steps
+    late (another subset phase, so never grouped with the first seven): every
+    frame equals its own session bit for bit, image and FP64 log-likelihood,
+    and a batch without groups gives the same bits."""
+    g = SMALL
+    norm = norm_of(g, 11)
+    spec = [(2, 4, True), (2, 4, False), (1, 4, True), (1, 4, True), (2, 4, False),
+            (1, 4, True), (1, 4, True), (1, 2, True), (2, 2, False), (1, 3, True)]
+    late = (1, 4, True)
+    data = [frame_data(g, 500 + k, bg=hb) for k, (_, _, hb) in enumerate(spec + [late])]
+    out = {}
+    for grp in (0, 1):
+        bp = ga.batch_params(storage=storage, max_frames=12, min_nsub=2, lanes=2, group=grp)
+        bt = gctx.batch(g, bp, norm)
+        bt.use_graph(graph)
+        for k, (ni, ns, _) in enumerate(spec):
+            assert bt.add(ni, ns, *data[k]) == k
+        bt.step()
+        bt.step()
+        assert bt.add(late[0], late[1], *data[-1]) == len(spec)
+        bt.run()
+        out[grp] = [(bt.get(k), bt.loglik(k)) for k in range(len(spec) + 1)]
+        bt.close()
+    for k, (ni, ns, _) in enumerate(spec + [late]):
+        x_ref, ll_ref = session_result(gctx, g, norm, *data[k], ni, ns, storage)
+        for grp in (0, 1):
+            same(out[grp][k][0], x_ref, "frame %d (S=%d) group=%d" % (k, ns, grp))
+            assert out[grp][k][1] == ll_ref
+
+
 def test_graph_replays_match_eager_each_step(gctx):
     """Every replay of the captured step against the same step launched
     eagerly, with eager library work and host synchronisation in between."""
@@ -173,14 +210,17 @@ def test_inveon_size_batch_fp16_uneven_subsets(gctx, orc):
     """The batched dynamic study at its size: three frames on the Inveon
     geometry (image 128x128x159, sinogram 128x160x4319) with 8, 16 and 24
     subsets and uneven counts, FP16 storage, one iteration each, advanced
-    together by the captured graph over three lanes. Every frame equals its
-    own session bit for bit (image and FP64 log-likelihood), and the S=8 frame
-    (with a non-integer background) equals the FP32 oracle run on the
-    FP16-rounded inputs: one S=8 iteration, about 15 s on 16 host threads."""
+    together by the captured graph over three lanes, plus a second S=8 frame
+    that shares the first one's backprojection launches. Every frame equals
+    its own session bit for bit (image and FP64 log-likelihood), and the first
+    S=8 frame (with a non-integer background) equals the FP32 oracle run on
+    the FP16-rounded inputs: one S=8 iteration, about 15 s on 16 host
+    threads."""
     g = C1
     nd = g.ndata
     norm = synth.norm_uniform(nd, seed=61)
-    spec = [(8, 3e6, True), (16, 1e7, False), (24, 1e5, True)]  # (nsub, counts, bg)
+    # (nsub, counts, bg)
+    spec = [(8, 3e6, True), (16, 1e7, False), (24, 1e5, True), (8, 1e6, False)]
     bp = ga.batch_params(storage=ga.STORE_F16, max_frames=4, min_nsub=8, lanes=3)
     bt = gctx.batch(g, bp, norm)
     data = []

@@ -142,8 +142,18 @@ def test_batch_footprint():
     bp16 = ga.batch_params(storage=ga.STORE_F16, max_frames=64, min_nsub=16, lanes=2)
     assert ga.batch_footprint(C1, bp3) > base > ga.batch_footprint(C1, bp16)
     for bad in (ga.batch_params(lanes=0), ga.batch_params(min_nsub=0),
-                ga.batch_params(max_frames=0)):
+                ga.batch_params(max_frames=0), ga.batch_params(group=4),
+                ga.batch_params(group=-1)):
         assert ga.batch_footprint(C1, bad) == ga.GFR_EINVAL
+    # frame groups: per lane, interleaved rows (twice the packed rows) and a
+    # numerator and denominator per extra member; group 0 means 3
+    bp1 = ga.batch_params(storage=ga.STORE_F16, max_frames=64, min_nsub=8, lanes=2, group=1)
+    bp2 = ga.batch_params(storage=ga.STORE_F16, max_frames=64, min_nsub=8, lanes=2, group=2)
+    bp3g = ga.batch_params(storage=ga.STORE_F16, max_frames=64, min_nsub=8, lanes=2, group=3)
+    rows = 8 * 20 * (C1.nsinogs + 2 * C1.nseg) * (C1.nrad + 2)  # {ratio, calib}, 20 angles
+    f1, f2 = ga.batch_footprint(C1, bp1), ga.batch_footprint(C1, bp2)
+    assert f2 - f1 == 2 * (2 * rows + 2 * 4 * C1.nvox)
+    assert ga.batch_footprint(C1, bp3g) - f2 == 2 * (2 * 4 * C1.nvox) == base - f2
 
 
 def test_read_sinogram_file_semantics(tmp_path, orc):
