@@ -352,8 +352,9 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
     return rfail(r, set_err(ctx, GFR_ENOMEM, "reconstruction staging allocation failed"));
   // Measured sinograms with a NaN or infinite value fail the request: the
   // reference would carry it through the ratio into every voxel the LOR
-  // touches (GFIRST.cu:423-426). The normalization needs no check: a
-  // non-finite value fails its > 0.01 test and masks the bin, as there.
+  // touches (GFIRST.cu:423-426). The normalization needs no check: it only
+  // selects the calibration mask through its > 0.01 test, where NaN masks
+  // the bin and +inf keeps it, as there.
   unsigned int* bad = r->d_maxbits;
   auto upload = [&](const float* host, void* dst) -> hipError_t {
     hipError_t ee = hipMemcpyAsync(stage, host, pl.sino_f32, hipMemcpyDefault, st);
