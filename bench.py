@@ -145,7 +145,11 @@ def measured_pipes(dom, tune):
                                                                     tune["back_waves"]))
     if e is None:
         return None
-    return {"unit": "vector-memory path (TA address, TD data)", "ta_busy": e["ta_busy"],
+    # VALU issue: a CU's four 16-lane SIMDs issue about one wave64 VALU
+    # instruction per cycle between them (4 cycles each)
+    return {"unit": "VALU issue and the vector-memory path (TA address, TD data)",
+            "valu_wave_instr_per_cu_cycle": e["valu_wave_instr_per_cu_cycle"],
+            "valu_issue_limit_per_cu_cycle": 1.0, "ta_busy": e["ta_busy"],
             "td_busy": e["td_busy"], "source": "profiles/k_back_c1_pipes.json"}
 
 
