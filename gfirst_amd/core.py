@@ -124,7 +124,8 @@ class Context:
     def set_tuning(self, g, nsub, back_zb=8, zsum_sg=8, back_lds=False, back_waves=0, rot_z=16,
                    back_bw=4):
         """Fix the launch shapes for (geometry, nsub) instead of autotuning them:
-        back_zb slices per backprojector thread (4, 5, 6, 8, 16), back_lds the
+        back_zb slices per backprojector thread (1 or 2 with cached reads; 4, 5,
+        6, 8, 16), back_lds the
         per-block LDS row window, back_waves 6 (8 slices, cached reads) or 0,
         zsum_sg sinograms per z-sum wave (4, 6, 8), rot_z planes per rotation
         block (4, 16, 32), back_bw waves per backprojector block (4, or 1 with

@@ -572,10 +572,14 @@ int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, const gfr_tuni
   GFR_NONNULL(ctx, in, "tuning");
   if (nsub < 1 || nsub > g->nang) return set_err(ctx, GFR_EINVAL, "nsub out of range");
   const int zb = in->back_zb, sg = in->zsum_sg;
-  if (!(zb == 4 || zb == 5 || zb == 6 || zb == 8 || zb == 16) || !(sg == 4 || sg == 6 || sg == 8))
-    return set_err(ctx, GFR_EINVAL, "back_zb must be 4, 5, 6, 8 or 16 and zsum_sg 4, 6 or 8");
+  if (!(zb == 1 || zb == 2 || zb == 4 || zb == 5 || zb == 6 || zb == 8 || zb == 16) ||
+      !(sg == 4 || sg == 6 || sg == 8))
+    return set_err(ctx, GFR_EINVAL,
+                   "back_zb must be 1, 2, 4, 5, 6, 8 or 16 and zsum_sg 4, 6 or 8");
   if (in->back_lds != 0 && in->back_lds != 1)
     return set_err(ctx, GFR_EINVAL, "back_lds must be 0 or 1");
+  if (zb < 4 && in->back_lds)
+    return set_err(ctx, GFR_EINVAL, "back_zb 1 or 2 only with cached reads (back_lds 0)");
   if (in->back_waves != 0 && !(in->back_waves == 6 && in->back_zb == 8 && !in->back_lds))
     return set_err(ctx, GFR_EINVAL, "back_waves: 0, or 6 with 8 slices and direct reads");
   if (in->rot_z != 4 && in->rot_z != 16 && in->rot_z != 32)

@@ -1150,6 +1150,8 @@ hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* nume
     }                                                                                           \
   }
   switch (tune.back_zb) {
+    case 1: GFR_BK(1, 0); break;
+    case 2: GFR_BK(2, 0); break;
     case 4: GFR_BK(4, 0); break;
     case 5: GFR_BK(5, 0); break;
     case 6: GFR_BK(6, 0); break;
@@ -1222,9 +1224,12 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
     }
   }
   // backprojector: (kind, slices, register budget, waves per block)
-  const int bcand[12][4] = {{0, 4, 0, 4}, {0, 5, 0, 4}, {0, 6, 0, 4}, {0, 8, 0, 4},
+  // (1 and 2 slices: small images, where each wave's sequential walk over
+  // the views, not the device's throughput, sets the time)
+  const int bcand[15][4] = {{0, 4, 0, 4}, {0, 5, 0, 4}, {0, 6, 0, 4}, {0, 8, 0, 4},
                             {0, 8, 6, 4}, {0, 16, 0, 4}, {0, 4, 0, 1}, {0, 6, 0, 1},
-                            {0, 8, 0, 1}, {1, 4, 0, 4},  {1, 8, 0, 4}, {1, 16, 0, 4}};
+                            {0, 8, 0, 1}, {1, 4, 0, 4},  {1, 8, 0, 4}, {1, 16, 0, 4},
+                            {0, 1, 0, 1}, {0, 2, 0, 1},  {0, 2, 0, 4}};
   for (const auto& c : bcand) {
     Tune tt = t;
     tt.back_lds = c[0];

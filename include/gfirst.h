@@ -130,7 +130,8 @@ int gfr_zero(gfr_ctx* ctx, void* dptr, size_t bytes, void* stream);
  * context's current choice. No counterpart in the reference (its launch
  * shapes are fixed, GPU:105-107, 166-171). */
 typedef struct gfr_tuning {
-  int back_zb;    /* slices per backprojector thread: 4, 5, 6, 8 or 16           */
+  int back_zb;    /* slices per backprojector thread: 1, 2 (back_lds 0), 4, 5, 6,
+                     8 or 16                                                    */
   int back_lds;   /* 1: per-block LDS window of each view's rows; 0: cached reads */
   int back_waves; /* 6: 8-slice direct kernel built for 6 waves/SIMD; else 0     */
   int zsum_sg;    /* sinograms per forward z-sum wave: 4, 6 or 8                 */

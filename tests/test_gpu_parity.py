@@ -104,7 +104,7 @@ def test_odd_geometry_operators(gctx, orc):
     valid = rng.uniform(size=nd) < 0.95
     for s, S in ((0, 1), (4, 5)):
         ne, de = orc.back(g, corr, np.where(valid, 1, -1000).astype(np.float32), s, S)
-        for zb in (4, 5, 6, 8, 16):
+        for zb in (1, 2, 4, 5, 6, 8, 16):
             for kind, blk in ((0, 4), (0, 1)):
                 set_back_shape(gctx, g, S, (zb, kind, 0, blk))
                 n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
@@ -118,7 +118,7 @@ def test_odd_geometry_operators(gctx, orc):
 # reads, 1 LDS window; register budget; waves per block)
 BACK_SHAPES = [(4, 0, 0, 4), (5, 0, 0, 4), (6, 0, 0, 4), (8, 0, 0, 4), (8, 0, 6, 4),
                (16, 0, 0, 4), (4, 0, 0, 1), (6, 0, 0, 1), (8, 0, 0, 1), (4, 1, 0, 4),
-               (8, 1, 0, 4), (16, 1, 0, 4)]
+               (8, 1, 0, 4), (16, 1, 0, 4), (1, 0, 0, 1), (2, 0, 0, 1), (2, 0, 0, 4)]
 
 
 def set_back_shape(gctx, g, S, shape):
@@ -258,7 +258,8 @@ def test_back_project_c1_subset_vs_oracle(gctx, orc):
     (4, 4, 0, 0, 32, 4), (5, 8, 0, 0, 16, 4), (6, 4, 0, 0, 4, 4), (8, 6, 0, 0, 4, 4),
     (16, 8, 0, 0, 16, 4), (8, 8, 0, 6, 4, 4), (4, 8, 1, 0, 16, 4), (6, 8, 1, 0, 32, 4),
     (8, 4, 1, 0, 32, 4), (16, 6, 1, 0, 4, 4), (4, 4, 0, 0, 16, 1), (6, 8, 0, 0, 4, 1),
-    (8, 6, 0, 0, 32, 1), (5, 8, 0, 0, 16, 1), (16, 4, 0, 0, 16, 1)])
+    (8, 6, 0, 0, 32, 1), (5, 8, 0, 0, 16, 1), (16, 4, 0, 0, 16, 1), (1, 8, 0, 0, 16, 1),
+    (2, 4, 0, 0, 4, 1), (2, 6, 0, 0, 32, 4)])
 def test_tile_shapes_do_not_change_results(gctx, orc, zb, sg, lds, bw, rz, blk):
     """Every autotuner candidate gives the oracle's bits (forward and back)."""
     g = C0
@@ -604,7 +605,8 @@ def test_tuning_rejects_invalid_shapes(gctx):
     before = gctx.tuning(g, 3)
     bad = [dict(back_zb=7), dict(zsum_sg=5), dict(rot_z=8), dict(back_lds=2),
            dict(back_waves=6, back_zb=6), dict(back_waves=6, back_lds=True),
-           dict(back_bw=2), dict(back_bw=1, back_lds=True), dict(back_bw=1, back_waves=6)]
+           dict(back_bw=2), dict(back_bw=1, back_lds=True), dict(back_bw=1, back_waves=6),
+           dict(back_zb=3), dict(back_zb=2, back_lds=True), dict(back_zb=1, back_waves=6)]
     for kw in bad:
         with pytest.raises(ga.GfrError):
             gctx.set_tuning(g, 3, **kw)
