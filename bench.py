@@ -876,12 +876,15 @@ def served_stream(ctx, cls, storage, sname, duration, rate, max_active, seed, bu
     t_end = time.perf_counter()
     infos = [(srv.info(rid), name, niter, counts) for rid, name, niter, counts in ids]
     failed = [i for i, _, _, _ in infos if i["state"] != ga.REQ_DONE]
-    lat = {}
+    lat, adm, run = {}, {}, {}
     passes = eq = 0.0
     for i, name, niter, counts in infos:
         if i["state"] != ga.REQ_DONE:
             continue
         lat.setdefault(name, []).append(i["t_done"] - i["t_submit"])
+        # admitted = session created (after any wait for a slot or memory)
+        adm.setdefault(name, []).append(i["t_admit"] - i["t_submit"])
+        run.setdefault(name, []).append(i["t_done"] - i["t_admit"])
         c = cls[name]
         npass = niter * c["nsub"] * (2 if c["mode"] >= 2 else 1)
         passes += npass
@@ -911,7 +914,8 @@ def served_stream(ctx, cls, storage, sname, duration, rate, max_active, seed, bu
                     "(mode 3, 2e6 VLOR events), counts log-U[1e5,5e7], 2-4 iterations, "
                     "%s storage, %.0f GB budget" % (rate, duration, sname, budget_gb),
         "p50_ms": q(allat, 50), "p99_ms": q(allat, 99), "max_ms": q(allat, 100),
-        "per_class": {k: {"n": len(v), "p50_ms": q(v, 50), "p99_ms": q(v, 99)}
+        "per_class": {k: {"n": len(v), "p50_ms": q(v, 50), "p99_ms": q(v, 99),
+                          "admit_p50_ms": q(adm[k], 50), "run_p50_ms": q(run[k], 50)}
                       for k, v in lat.items()},
         "requests": len(ids), "failed": len(failed),
         "subset_iters_per_s": round(passes / span, 2),
