@@ -43,7 +43,7 @@ class GfrReconParams(C.Structure):
 class GfrTuning(C.Structure):
     _fields_ = [("back_zb", C.c_int), ("back_lds", C.c_int), ("back_waves", C.c_int),
                 ("zsum_sg", C.c_int), ("rot_z", C.c_int), ("back_bw", C.c_int),
-                ("ms_back", C.c_float), ("ms_zsum", C.c_float)]
+                ("ms_back", C.c_float), ("ms_zsum", C.c_float), ("back_ahead", C.c_int)]
 
 
 class GfrBatchParams(C.Structure):

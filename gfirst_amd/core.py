@@ -122,16 +122,18 @@ class Context:
         return d
 
     def set_tuning(self, g, nsub, back_zb=8, zsum_sg=8, back_lds=False, back_waves=0, rot_z=16,
-                   back_bw=4):
+                   back_bw=4, back_ahead=1):
         """Fix the launch shapes for (geometry, nsub) instead of autotuning them:
         back_zb slices per backprojector thread (1 or 2 with cached reads; 4, 5,
         6, 8, 16), back_lds the
         per-block LDS row window, back_waves 6 (8 slices, cached reads) or 0,
         zsum_sg sinograms per z-sum wave (4, 6, 8), rot_z planes per rotation
         block (4, 16, 32), back_bw waves per backprojector block (4, or 1 with
-        cached reads and back_waves 0). Applies to sessions created
-        afterwards; no shape changes any result."""
-        t = _lib.GfrTuning(back_zb, int(back_lds), back_waves, zsum_sg, rot_z, back_bw, 0.0, 0.0)
+        cached reads and back_waves 0), back_ahead views whose rows are loaded
+        before the first is summed (2 or 4 with one slice and cached reads).
+        Applies to sessions created afterwards; no shape changes any result."""
+        t = _lib.GfrTuning(back_zb, int(back_lds), back_waves, zsum_sg, rot_z, back_bw, 0.0, 0.0,
+                           back_ahead)
         _check(self.h, lib().gfr_tuning_set(self.h, C.byref(g.c()), nsub, C.byref(t)))
 
     # -- operators (one per reference wrapper) ---------------------------------

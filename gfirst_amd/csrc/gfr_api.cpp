@@ -563,6 +563,7 @@ int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, gfr_tuning* ou
   out->back_bw = t.back_bw;
   out->ms_back = t.ms_back;
   out->ms_zsum = t.ms_zsum;
+  out->back_ahead = t.back_ahead;
   return t.tuned ? 1 : 0;
 }
 
@@ -586,6 +587,11 @@ int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, const gfr_tuni
     return set_err(ctx, GFR_EINVAL, "rot_z must be 4, 16 or 32");
   if (in->back_bw != 4 && !(in->back_bw == 1 && !in->back_lds && in->back_waves == 0))
     return set_err(ctx, GFR_EINVAL, "back_bw: 4, or 1 with cached reads and back_waves 0");
+  const int ahead = in->back_ahead == 0 ? 1 : in->back_ahead;
+  if (!(ahead == 1 || ahead == 2 || ahead == 4) ||
+      (ahead > 1 && !(zb == 1 && !in->back_lds && in->back_waves == 0)))
+    return set_err(ctx, GFR_EINVAL,
+                   "back_ahead: 1 (or 0), or 2 or 4 with back_zb 1 and cached reads");
   std::lock_guard<std::mutex> lk(ctx->mu);
   Tune& t = gd->tune[nsub];
   t.back_zb = in->back_zb;
@@ -594,6 +600,7 @@ int gfr_tuning_set(gfr_ctx* ctx, const gfr_geometry* g, int nsub, const gfr_tuni
   t.zsum_sg = in->zsum_sg;
   t.rot_z = in->rot_z;
   t.back_bw = in->back_bw;
+  t.back_ahead = ahead;
   t.tuned = true;
   t.ms_back = t.ms_zsum = 0.f;
   return GFR_OK;

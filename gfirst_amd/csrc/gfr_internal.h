@@ -52,6 +52,7 @@ struct Tune {
   int back_waves = 0;     // register budget for this many waves per SIMD (0: compiler's)
   int rot_z = 16;         // planes per rotation block: 4, 16 or 32
   int back_bw = 4;        // waves per direct backprojector block: 4 (16x16) or 1 (8x8)
+  int back_ahead = 1;     // 1-slice shapes: views whose rows are loaded ahead (1, 2, 4)
   int multi_zb = 0;       // slices of the multi-frame backprojector: 4 or 6 (0: not measured)
   bool tuned = false;
   float ms_back = 0.f, ms_zsum = 0.f;

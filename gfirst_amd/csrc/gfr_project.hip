@@ -941,6 +941,88 @@ __global__ __launch_bounds__(256) void k_back_multi(DevGeom G, const float* __re
   }
 }
 
+// One slice per thread for small images, where the grid is a few hundred
+// waves and each wave's walk over the views is a chain of memory round trips:
+// the rows of NA consecutive views are all requested before the first of them
+// is summed, so the chain waits once per NA views. With one slice the
+// per-slice formula of accumulate_view (plane position w0, its floor and
+// fraction, rows ii and ii + 1) is the whole computation, and the views are
+// summed in the same order, so the sums are those of k_back_direct<1>.
+template <int NA, int BW>
+__global__ __launch_bounds__(64 * BW) void k_back_ahead(DevGeom G, const float2* __restrict__ c2,
+                                                        unsigned c2_bytes, float* __restrict__ numer,
+                                                        float* __restrict__ denom, int subset,
+                                                        int nsub, const int* __restrict__ dsub) {
+  static_assert(BW == 1 || BW == 4, "blocks of 1 or 4 waves");
+  constexpr int kTile = BW == 4 ? 16 : 8;
+  if (dsub) subset = *dsub;
+  const int nav = (G.nang - subset + nsub - 1) / nsub;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ix = blockIdx.x * kTile + (wv & 1) * 8 + (lane & 7);
+  const int iy = blockIdx.y * kTile + (wv >> 1) * 8 + (lane >> 3);
+  const int zb = blockIdx.z;
+  const bool act = ix < G.res && iy < G.res;
+  const __amdgpu_buffer_rsrc_t rc2 = rsrc(c2, c2_bytes);
+  const float xv = (float)ix - G.RESM, yv = (float)iy - G.RESM;
+  const unsigned rowb = (unsigned)packed_row_len(G.nrad) * 8u;
+  const unsigned nr = (unsigned)(G.nsinogs + 2 * G.nseg);
+  const float fz0 = (float)zb;
+  float n = 0.f, d = 0.f;
+  for (int si = 0; si < G.nseg; si++) {
+    const SegEntry se = G.seg[si];
+    const float fzm = (float)se.zmin, fzhi = (float)(G.nzs - se.zmin);
+    const unsigned segrow = (unsigned)(se.nza + 2 * si + 1 - se.zmin);
+    for (int a0 = 0; a0 < nav; a0 += NA) {
+      float4 p0[NA], p1[NA];
+      float au[NA], al[NA];
+      bool inr[NA], used[NA];
+#pragma unroll
+      for (int u = 0; u < NA; u++) {
+        const int a = a0 + u;
+        used[u] = false;
+        inr[u] = false;
+        au[u] = al[u] = 0.f;
+        if (a < nav) {
+          const int it = subset + a * nsub;
+          const LaneView v = lane_view(G, xv, yv, G.sin_t[it], G.cos_t[it], se.fzb, fz0, fz0,
+                                       fzm, fzhi, act);
+          used[u] = __builtin_amdgcn_ballot_w64(v.any) != 0;
+          if (used[u]) {
+            const float w = v.w0;
+            inr[u] = v.any && (zb < G.nzs) && w >= fzm && w < fzhi;
+            const float t = w - 0.5f;
+            const float f = floorf(t);
+            al[u] = t - f;
+            au[u] = v.au;
+            const unsigned rb = ((unsigned)a * nr + segrow + (unsigned)(int)f) * rowb +
+                                (unsigned)(v.iq + 1) * 8u;
+            p0[u] = ld_f4(rc2, rb, 0);
+            p1[u] = ld_f4(rc2, rb, rowb);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NA; u++) {
+        if (!used[u]) continue;  // wave-uniform: the view misses the wave
+        const float c0 = lerp1(p0[u].x, p0[u].z, au[u]), c1 = lerp1(p1[u].x, p1[u].z, au[u]);
+        const float q0 = lerp1(p0[u].y, p0[u].w, au[u]), q1 = lerp1(p1[u].y, p1[u].w, au[u]);
+        float A = inr[u] ? lerp1(c0, c1, al[u]) : 0.f;
+        float B = inr[u] ? lerp1(q0, q1, al[u]) : 0.f;
+        if (B < 0.f || A < 0.f) {  // gaps (GFIRST_GPU_Kernels.cu:157)
+          A = 0.f;
+          B = 0.f;
+        }
+        n += A;
+        d += B;
+      }
+    }
+  }
+  if (!act || zb >= G.nzs) return;
+  const size_t iV = ((size_t)zb * G.res + iy) * G.res + ix;
+  numer[iV] = n;
+  denom[iV] = d;
+}
+
 inline int nav_of(const DevGeom& G, int subset, int nsub) {
   return (G.nang - subset + nsub - 1) / nsub;
 }
@@ -1149,6 +1231,19 @@ hipError_t launch_back_packed(const DevGeom& G, const float2* corr2, float* nume
                          nsub, dsub);                                                           \
     }                                                                                           \
   }
+  if (tune.back_ahead > 1 && tune.back_zb == 1 && !tune.back_lds && tune.back_waves == 0) {
+    const unsigned n8 = (G.res + 7) / 8;
+#define GFR_BA(NA, BW)                                                                            \
+  hipLaunchKernelGGL((k_back_ahead<NA, BW>), BW == 4 ? dim3(nbx, nbx, G.nzs) : dim3(n8, n8, G.nzs), \
+                     dim3(64 * BW), 0, st, G, corr2, (unsigned)bytes, numer, denom, subset, nsub, dsub)
+    if (tune.back_ahead == 4) {
+      if (tune.back_bw == 1) GFR_BA(4, 1); else GFR_BA(4, 4);
+    } else {
+      if (tune.back_bw == 1) GFR_BA(2, 1); else GFR_BA(2, 4);
+    }
+#undef GFR_BA
+    return hipGetLastError();
+  }
   switch (tune.back_zb) {
     case 1: GFR_BK(1, 0); break;
     case 2: GFR_BK(2, 0); break;
@@ -1226,16 +1321,19 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
   // backprojector: (kind, slices, register budget, waves per block)
   // (1 and 2 slices: small images, where each wave's sequential walk over
   // the views, not the device's throughput, sets the time)
-  const int bcand[15][4] = {{0, 4, 0, 4}, {0, 5, 0, 4}, {0, 6, 0, 4}, {0, 8, 0, 4},
-                            {0, 8, 6, 4}, {0, 16, 0, 4}, {0, 4, 0, 1}, {0, 6, 0, 1},
-                            {0, 8, 0, 1}, {1, 4, 0, 4},  {1, 8, 0, 4}, {1, 16, 0, 4},
-                            {0, 1, 0, 1}, {0, 2, 0, 1},  {0, 2, 0, 4}};
+  // (views whose rows are loaded ahead: 1-slice shapes only)
+  const int bcand[18][5] = {{0, 4, 0, 4, 1}, {0, 5, 0, 4, 1}, {0, 6, 0, 4, 1}, {0, 8, 0, 4, 1},
+                            {0, 8, 6, 4, 1}, {0, 16, 0, 4, 1}, {0, 4, 0, 1, 1}, {0, 6, 0, 1, 1},
+                            {0, 8, 0, 1, 1}, {1, 4, 0, 4, 1},  {1, 8, 0, 4, 1}, {1, 16, 0, 4, 1},
+                            {0, 1, 0, 1, 1}, {0, 2, 0, 1, 1},  {0, 2, 0, 4, 1}, {0, 1, 0, 1, 2},
+                            {0, 1, 0, 1, 4}, {0, 1, 0, 4, 4}};
   for (const auto& c : bcand) {
     Tune tt = t;
     tt.back_lds = c[0];
     tt.back_zb = c[1];
     tt.back_waves = c[2];
     tt.back_bw = c[3];
+    tt.back_ahead = c[4];
     const float ms = time_it([&] {
       return launch_back_packed(G, corr2, num, den, 0, nsub, nullptr, tt, st);
     });
@@ -1245,6 +1343,7 @@ hipError_t tune_shapes(const DevGeom& G, int nsub, const FwdOut& fo, const float
       t.back_zb = tt.back_zb;
       t.back_waves = tt.back_waves;
       t.back_bw = tt.back_bw;
+      t.back_ahead = tt.back_ahead;
     }
   }
   (void)hipEventDestroy(e0);

@@ -140,6 +140,9 @@ typedef struct gfr_tuning {
                      1 only with back_lds 0 and back_waves 0                    */
   float ms_back;  /* measured time of the chosen backprojector (0 if not measured) */
   float ms_zsum;  /* measured time of the chosen forward projector              */
+  int back_ahead; /* views whose rows are loaded before the first is summed: 1 (or
+                     0), 2 or 4; above 1 only with back_zb 1, back_lds 0 and
+                     back_waves 0                                               */
 } gfr_tuning;
 /* Returns 1 if the shapes were measured, 0 if defaults. */
 int gfr_tuning_get(gfr_ctx* ctx, const gfr_geometry* g, int nsub, gfr_tuning* t);

@@ -105,10 +105,12 @@ def test_odd_geometry_operators(gctx, orc):
     for s, S in ((0, 1), (4, 5)):
         ne, de = orc.back(g, corr, np.where(valid, 1, -1000).astype(np.float32), s, S)
         for zb in (1, 2, 4, 5, 6, 8, 16):
-            for kind, blk in ((0, 4), (0, 1)):
-                set_back_shape(gctx, g, S, (zb, kind, 0, blk))
+            for kind, blk, ahead in ((0, 4, 1), (0, 1, 1), (0, 1, 4), (0, 4, 2)):
+                if ahead > 1 and zb != 1:
+                    continue
+                set_back_shape(gctx, g, S, (zb, kind, 0, blk, ahead))
                 n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
-                what = "odd zb=%d kind=%d blk=%d S=%d" % (zb, kind, blk, S)
+                what = "odd zb=%d kind=%d blk=%d ahead=%d S=%d" % (zb, kind, blk, ahead, S)
                 close(n, ne, rtol=0, what="numer " + what)
                 close(d, de, rtol=0, what="denom " + what)
     close(gctx.blur_xy(g, x, 1.5), orc.blur_xy(g, x, 1.5), rtol=0, what="blur odd")
@@ -118,12 +120,21 @@ def test_odd_geometry_operators(gctx, orc):
 # reads, 1 LDS window; register budget; waves per block)
 BACK_SHAPES = [(4, 0, 0, 4), (5, 0, 0, 4), (6, 0, 0, 4), (8, 0, 0, 4), (8, 0, 6, 4),
                (16, 0, 0, 4), (4, 0, 0, 1), (6, 0, 0, 1), (8, 0, 0, 1), (4, 1, 0, 4),
-               (8, 1, 0, 4), (16, 1, 0, 4), (1, 0, 0, 1), (2, 0, 0, 1), (2, 0, 0, 4)]
+               (8, 1, 0, 4), (16, 1, 0, 4), (1, 0, 0, 1), (2, 0, 0, 1), (2, 0, 0, 4),
+               (1, 0, 0, 1, 2), (1, 0, 0, 1, 4), (1, 0, 0, 4, 4)]
 
 
 def set_back_shape(gctx, g, S, shape):
-    zb, kind, waves, blk = shape
-    gctx.set_tuning(g, S, zb, 8, back_lds=kind == 1, back_waves=waves, back_bw=blk)
+    """shape: (slices, kind, register budget, waves per block[, views loaded ahead])"""
+    zb, kind, waves, blk = shape[:4]
+    ahead = shape[4] if len(shape) > 4 else 1
+    gctx.set_tuning(g, S, zb, 8, back_lds=kind == 1, back_waves=waves, back_bw=blk,
+                    back_ahead=ahead)
+
+
+def shape_name(shape):
+    return "zb=%d kind=%d waves=%d blk=%d" % shape[:4] + (
+        " ahead=%d" % shape[4] if len(shape) > 4 else "")
 
 
 def test_reference_geometry_operators(gctx, orc):
@@ -156,7 +167,7 @@ def test_reference_geometry_operators(gctx, orc):
     for shape in BACK_SHAPES:
         set_back_shape(gctx, g, S, shape)
         n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
-        what = "ref zb=%d kind=%d waves=%d blk=%d" % shape
+        what = "ref " + shape_name(shape)
         close(n, ne, what="numer " + what)
         close(d, de, what="denom " + what)
     gctx.set_tuning(g, S, 8, 8)
@@ -223,11 +234,11 @@ def test_back_project_structured_mask(gctx, orc, g, s, S):
     valid.reshape(-1)[dead] = False
     valid = valid.reshape(-1)
     ne, de = orc.back(g, corr, np.where(valid, 1, -1000).astype(np.float32), s, S)
-    for shape in ((6, 0, 0, 4), (8, 1, 0, 4), (8, 0, 0, 1), (4, 0, 0, 4)):
+    for shape in ((6, 0, 0, 4), (8, 1, 0, 4), (8, 0, 0, 1), (4, 0, 0, 4), (1, 0, 0, 1, 4)):
         set_back_shape(gctx, g, S, shape)
         n, d = gctx.back_project(g, corr, valid.astype(np.uint8), s, S)
-        close(n, ne, rtol=0, what="numer zb=%d kind=%d waves=%d blk=%d" % shape)
-        close(d, de, rtol=0, what="denom zb=%d kind=%d waves=%d blk=%d" % shape)
+        close(n, ne, rtol=0, what="numer " + shape_name(shape))
+        close(d, de, rtol=0, what="denom " + shape_name(shape))
     gctx.set_tuning(g, S, 8, 8)
 
 
@@ -244,11 +255,11 @@ def test_back_project_c1_subset_vs_oracle(gctx, orc):
     close(d, de, what="denom c1")
     # the LDS-window backprojector (windows at full tilt) and the one-wave
     # blocks of the cached-read kernel on the same views
-    for shape in ((8, 1, 0, 4), (6, 0, 0, 1)):
+    for shape in ((8, 1, 0, 4), (6, 0, 0, 1), (1, 0, 0, 4, 4)):
         set_back_shape(gctx, g, 80, shape)
         n2, d2 = gctx.back_project(g, corr, valid.astype(np.uint8), 5, 80)
-        close(n2, ne, rtol=0, what="numer c1 zb=%d kind=%d waves=%d blk=%d" % shape)
-        close(d2, de, rtol=0, what="denom c1 zb=%d kind=%d waves=%d blk=%d" % shape)
+        close(n2, ne, rtol=0, what="numer c1 " + shape_name(shape))
+        close(d2, de, rtol=0, what="denom c1 " + shape_name(shape))
     gctx.set_tuning(g, 80, 8, 8)
 
 
@@ -606,7 +617,9 @@ def test_tuning_rejects_invalid_shapes(gctx):
     bad = [dict(back_zb=7), dict(zsum_sg=5), dict(rot_z=8), dict(back_lds=2),
            dict(back_waves=6, back_zb=6), dict(back_waves=6, back_lds=True),
            dict(back_bw=2), dict(back_bw=1, back_lds=True), dict(back_bw=1, back_waves=6),
-           dict(back_zb=3), dict(back_zb=2, back_lds=True), dict(back_zb=1, back_waves=6)]
+           dict(back_zb=3), dict(back_zb=2, back_lds=True), dict(back_zb=1, back_waves=6),
+           dict(back_zb=2, back_ahead=2), dict(back_zb=1, back_ahead=3),
+           dict(back_zb=1, back_ahead=4, back_lds=True)]
     for kw in bad:
         with pytest.raises(ga.GfrError):
             gctx.set_tuning(g, 3, **kw)
