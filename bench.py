@@ -829,7 +829,7 @@ def c4_inputs(ctx, c, counts, seed):
 
 
 def served_stream(ctx, cls, storage, sname, duration, rate, max_active, seed, budget_gb,
-                  error_report=False, inflight=2):
+                  error_report=False, inflight=2, admitters=4):
     """Plays the served stream: Poisson arrivals at `rate`/s for `duration` s
     of the c4 mix (classes from c4_setup), one untimed warm-up request per
     class, then every request waited for; latency = submission to the image
@@ -849,7 +849,7 @@ def served_stream(ctx, cls, storage, sname, duration, rate, max_active, seed, bu
         arrivals.append((ta, str(name), counts, niter))
     log("serve: %d arrivals over %.0fs (rate %.1f/s)" % (len(arrivals), duration, rate))
     # untimed warm-up: one request of each class (tile shapes, code paths)
-    srv = ctx.server(max_active=max_active, inflight=inflight)
+    srv = ctx.server(max_active=max_active, inflight=inflight, admitters=admitters)
     for k, name in enumerate(cls):
         c = cls[name]
         d2, t2 = c4_inputs(ctx, c, 1e6, 10 ** 6 + k)
@@ -925,6 +925,7 @@ def served_stream(ctx, cls, storage, sname, duration, rate, max_active, seed, bu
         "alone_equals_mix": same,
         "max_active": max_active,
         "inflight": inflight,
+        "admitters": admitters,
         "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0),
     }
     if err is not None:
@@ -943,7 +944,8 @@ def run_c4(args):
     cls = c4_setup(ctx, args.seed)
     log("c4: classes ready in %.1fs" % (time.perf_counter() - t))
     sv = served_stream(ctx, cls, storage, sname, args.duration, args.rate, args.max_active,
-                       args.seed, args.budget_gb, args.error_report, args.inflight)
+                       args.seed, args.budget_gb, args.error_report, args.inflight,
+                       args.admitters)
     res = {
         "metric": "p50/p99 recon latency under load",
         "value": sv["p50_ms"],
@@ -1055,6 +1057,8 @@ def main():
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--inflight", type=int, default=2,
                     help="served stream: passes of a session queued ahead of the device")
+    ap.add_argument("--admitters", type=int, default=4,
+                    help="served stream: threads creating sessions")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-active", type=int, default=16)
     ap.add_argument("--hw-queues", type=int, default=16, help="c4: GPU_MAX_HW_QUEUES (<= 32)")
