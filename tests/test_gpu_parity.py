@@ -531,15 +531,21 @@ def test_errors_are_status_codes(gctx):
     small.close()
 
 
-def test_joint_inveon_first_triples_passes_vs_oracle(gctx, orc):
-    """The joint reconstruction (mode 3) at the Inveon size, 860k VLOR records
-    from 2e6 triple events: the session's first 18 passes (16 doubles
-    subsets, then triples subsets 0 and 1 with VLOR assignment, projection
-    cache, bilateral filter and blend) against the oracle following the
-    reference's loop (GFIRST.cu:324-527) pass by pass, bit for bit.
-    About 30 s of oracle work on 16 host threads."""
+@pytest.mark.parametrize("g,S,events,span", [(C1, 16, 2000000, False),
+                                              (REF, 12, 1000000, True)],
+                         ids=["inveon", "reference-geometry-2-iterations"])
+def test_joint_first_triples_passes_vs_oracle(gctx, orc, g, S, events, span):
+    """The joint reconstruction (mode 3) with VLOR records binned from triple
+    events, pass by pass against the oracle following the reference's loop
+    (GFIRST.cu:324-527), bit for bit. At the Inveon size (860k records from
+    2e6 events): the first S + 2 passes (S doubles subsets, then triples
+    subsets 0 and 1 with VLOR assignment, projection cache, bilateral filter
+    and blend). At the reference's compiled geometry (span 11, NRAD 147, 15
+    segments; GFIRST.h:4-16): the first 2S + 2 passes of two iterations, so
+    iteration 1's triples passes run on the session's second stream beside
+    iteration 2's first doubles passes, each still seeing its sequential
+    inputs. About 30-40 s of oracle work each on 16 host threads."""
     from gfirst_amd import listmode, synth
-    g, S = C1, 16
     nd = g.ndata
 
     def expected(img):
@@ -553,13 +559,17 @@ def test_joint_inveon_first_triples_passes_vs_oracle(gctx, orc):
     d2 = synth.as_file_pair(rng.poisson(yDe * (1e7 / yDe.sum())).astype(np.float32))
     t2 = synth.as_file_pair(rng.poisson(yTe * (2e5 / yTe.sum())).astype(np.float32))
     norm = synth.norm_uniform(nd, seed=5)
-    rec = listmode.bin_hits(g, listmode.triple_events(g, chB, 2000000, seed=8), 80.5)
+    rec = listmode.bin_hits(g, listmode.triple_events(g, chB, events, seed=8), 80.5)
     rec = rec[(rec[:, 0] > 0) & (rec[:, 1] > 0)]
-    r = gctx.recon(g, ga.recon_params(mode=3, niter=1, nsub=S), d2, t2, norm, rec)
-    for _ in range(S + 2):
+    assert len(rec) > events // 10
+    nT = S if span else 2               # triples passes compared
+    nD = S + 2 if span else S           # doubles passes compared
+    r = gctx.recon(g, ga.recon_params(mode=3, niter=2 if span else 1, nsub=S), d2, t2, norm,
+                   rec)
+    for _ in range(nD + nT):
         r.step()
     c = r.cursor()
-    assert (c["type"], c["subset"], c["passes"]) == (1, 2, S + 2)
+    assert (c["type"], c["subset"], c["passes"]) == ((0, 2) if span else (1, 2)) + (nD + nT,)
     xD, xT = r.get(ga.OUT_XD), r.get(ga.OUT_XT)
     r.close()
 
@@ -571,20 +581,26 @@ def test_joint_inveon_first_triples_passes_vs_oracle(gctx, orc):
     eD, eT = orc.cylinder(g, 1.0), orc.cylinder(g, 1.0)
     ptc = orc.forward(g, orc.blur_xy(g, eT, 1.5), 0, 1)
     dv = np.zeros(nd, np.float32)
-    for s in range(S):
+    def doubles(eD, s):
         proj = orc.forward(g, orc.blur_xy(g, eD, 1.5), s, S)
         num, den = orc.back(g, orc.ratio(g, proj, yD, bD, s, S), cal, s, S)
-        eD = orc.update(g, eD, num, den, 1.5)
-    for s in range(2):
+        return orc.update(g, eD, num, den, 1.5)
+
+    for s in range(S):
+        eD = doubles(eD, s)
+    guide = eD  # iteration 1's doubles image guides iteration 1's triples passes
+    for s in range(nT):
         proj = orc.forward(g, orc.blur_xy(g, eT, 1.5), s, S)
         cur = ptc.reshape(g.nsinogs, g.nang, g.nrad)
         cur[:, s::S, :] = proj.reshape(g.nsinogs, g.nang, g.nrad)[:, s::S, :]
         dv = orc.vlor_assign(g, rec, blankt, ptc, s, S, dv)
         num, den = orc.back(g, orc.ratio(g, proj, dv * res, bT, s, S), cal, s, S)
         eT = orc.update(g, eT, num, den, 1.5)
-        eT = orc.blend(g, eT, orc.jbf(g, eT, eD, orc.vmax(eD)))
-    close(xD, eD, rtol=0, what="joint c1 xD after 16 doubles passes")
-    close(xT, eT, rtol=0, what="joint c1 xT after 2 triples passes")
+        eT = orc.blend(g, eT, orc.jbf(g, eT, guide, orc.vmax(guide)))
+    for s in range(nD - S):  # iteration 2's first doubles passes
+        eD = doubles(eD, s)
+    close(xD, eD, rtol=0, what="joint xD after %d doubles passes" % nD)
+    close(xT, eT, rtol=0, what="joint xT after %d triples passes" % nT)
     assert np.array_equal(xD, eD) and np.array_equal(xT, eT)
 
 
