@@ -214,6 +214,42 @@ def test_nan_inputs_and_cancellation_are_isolated(gctx):
     assert np.array_equal(res["nannorm"], exp) and np.all(np.isfinite(exp))
 
 
+def test_cancel_running_joint_request(gctx):
+    """A joint request (mode 3: triples passes on the session's second
+    stream) cancelled while both of its streams have work queued stops
+    between passes with GFR_ECANCELED; the requests beside it and a joint
+    request admitted afterwards come back with their bits alone."""
+    g = C0
+    srv = gctx.server(max_active=3, inflight=2)
+    long_p = ga.recon_params(mode=3, niter=30, nsub=16)
+    p3 = ga.recon_params(mode=3, niter=2, nsub=4)
+    p2 = ga.recon_params(mode=2, niter=2, nsub=4)
+    jl = request(g, 951, 3, 3e5)
+    j3 = request(g, 953, 3, 3e5)
+    j2 = request(g, 955, 2, 3e5)
+    rid_long = srv.submit(g, long_p, jl[0], jl[1], jl[2], jl[3])
+    rid2 = srv.submit(g, p2, j2[0], j2[1], j2[2], None)
+    t0 = time.time()
+    # past the first iteration, so triples passes of one iteration and
+    # doubles passes of the next are both in flight
+    while srv.info(rid_long)["passes"] < 40 and time.time() - t0 < 60:
+        time.sleep(0.001)
+    assert srv.info(rid_long)["state"] == ga.REQ_ACTIVE
+    srv.cancel(rid_long)
+    rid3 = srv.submit(g, p3, j3[0], j3[1], j3[2], j3[3])
+    for rid in (rid_long, rid2, rid3):
+        assert srv.wait(rid, 120.0) in (ga.REQ_DONE, ga.REQ_FAILED)
+    info = srv.info(rid_long)
+    assert info["state"] == ga.REQ_FAILED and info["error"] == ga.GFR_ECANCELED
+    assert 40 <= info["passes"] < 30 * 2 * 16
+    got2, got3 = srv.result(rid2), srv.result(rid3)
+    srv.close()
+    exp2 = alone(gctx, g, p2, j2[0], j2[1], j2[2], None)
+    exp3 = alone(gctx, g, p3, j3[0], j3[1], j3[2], j3[3])
+    for got, exp, what in ((got2, exp2, "mode 2"), (got3, exp3, "mode 3")):
+        assert np.array_equal(got[0], exp[0]) and np.array_equal(got[1], exp[1]), what
+
+
 def test_destroy_with_work_in_flight(gctx):
     srv = gctx.server(max_active=2)
     ids = []
