@@ -128,14 +128,15 @@ def test_frame_groups_equal_sessions(gctx, storage, graph):
 
 def test_graph_replays_match_eager_each_step(gctx):
     """Every replay of the captured step against the same step launched
-    eagerly, with eager library work and host synchronisation in between."""
+    eagerly, with eager library work and host synchronisation in between;
+    three of the frames share their backprojection (one group)."""
     g = C0
     norm = norm_of(g, 5)
-    bp = ga.batch_params(storage=ga.STORE_F16, max_frames=4, min_nsub=4, lanes=2)
+    bp = ga.batch_params(storage=ga.STORE_F16, max_frames=6, min_nsub=4, lanes=2)
     bg_ = gctx.batch(g, bp, norm)
     be = gctx.batch(g, bp, norm)
     be.use_graph(False)
-    spec = [(1, 4), (1, 8), (1, 16)]
+    spec = [(1, 4), (1, 8), (1, 16), (1, 4), (1, 4)]
     for k, (ni, ns) in enumerate(spec):
         y, b = frame_data(g, 200 + k, counts=3e5)
         bg_.add(ni, ns, y, b)
