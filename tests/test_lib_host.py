@@ -5,6 +5,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -188,3 +189,11 @@ def test_read_vlor_file_semantics(tmp_path, orc):
 def test_cli_usage_without_gpu():
     out = subprocess.run([_lib.CLI_PATH], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0 and "1 Input  --> Projection" in out.stdout
+
+
+def test_bench_refuses_more_than_one_gpu():
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                         capture_output=True, text=True, timeout=300, cwd=root)
+    assert out.returncode == 2 and out.stdout == ""
+    assert "one GPU" in out.stderr and "--gpus 2" in out.stderr
