@@ -67,6 +67,25 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
   return x < rem ? x * (nx + 1) + k : rem * (nx + 1) + (x - rem) * nx + k;
 }
 
+// The same round-robin for a 3-D grid of (x tile, y tile, z slab) blocks,
+// linearised x fastest. With an even x count and a y count divisible by 4,
+// XCD k takes every z slab of the tiles of one (nx/2) x (ny/4) region
+// (x half k & 1, y quarter k >> 1), so its L2 holds only the rows that
+// region projects to; other grids keep the hardware order. Bijective.
+__device__ __forceinline__ void xcd_region(int& bx, int& by, int& bz) {
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  bz = blockIdx.z;
+  if ((gridDim.x & 1u) == 0 && (gridDim.y & 3u) == 0) {
+    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const unsigned k = b & 7u, j = b >> 3;
+    const unsigned hx = gridDim.x >> 1, hy = gridDim.y >> 2, t = j % (hx * hy);
+    bz = (int)(j / (hx * hy));
+    bx = (int)((k & 1u) * hx + t % hx);
+    by = (int)((k >> 1) * hy + t / hx);
+  }
+}
+
 // Packed {ratio, calib} rows of a subset for the backprojector. Per angle of
 // the subset and per segment, the segment's planes zmin..NZS-zmin-1 are
 // preceded and followed by one extra row, so that every plane a voxel can

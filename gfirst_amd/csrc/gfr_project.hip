@@ -712,9 +712,11 @@ __global__ __launch_bounds__(64 * BW, MINW > 0 ? MINW : 1) void k_back_direct(
   if (dsub) subset = *dsub;
   const int nav = (G.nang - subset + nsub - 1) / nsub;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int ix = blockIdx.x * kTile + (wv & 1) * 8 + (lane & 7);
-  const int iy = blockIdx.y * kTile + (wv >> 1) * 8 + (lane >> 3);
-  const int zb = blockIdx.z * ZB;
+  int bx, by, bz;
+  xcd_region(bx, by, bz);
+  const int ix = bx * kTile + (wv & 1) * 8 + (lane & 7);
+  const int iy = by * kTile + (wv >> 1) * 8 + (lane >> 3);
+  const int zb = bz * ZB;
   const bool act = ix < G.res && iy < G.res;
   const __amdgpu_buffer_rsrc_t rc2 = rsrc(c2, c2_bytes);
   const float xv = (float)ix - G.RESM, yv = (float)iy - G.RESM;
@@ -895,9 +897,11 @@ __global__ __launch_bounds__(256) void k_back_multi(DevGeom G, const float* __re
   if (dsub) subset = *dsub;
   const int nav = (G.nang - subset + nsub - 1) / nsub;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int ix = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
-  const int iy = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
-  const int zb = blockIdx.z * ZB;
+  int bx, by, bz;
+  xcd_region(bx, by, bz);
+  const int ix = bx * 16 + (wv & 1) * 8 + (lane & 7);
+  const int iy = by * 16 + (wv >> 1) * 8 + (lane >> 3);
+  const int zb = bz * ZB;
   const bool act = ix < G.res && iy < G.res;
   const __amdgpu_buffer_rsrc_t rc4 = rsrc(c4, c4_bytes);
   const float xv = (float)ix - G.RESM, yv = (float)iy - G.RESM;
