@@ -9,7 +9,7 @@ import pytest
 
 import gfirst_amd as ga
 from gfirst_amd import synth
-from gfirst_amd.geometry import C0, C1, SMALL
+from gfirst_amd.geometry import C0, C1, REF, SMALL
 
 pytestmark = pytest.mark.gpu
 
@@ -124,6 +124,29 @@ def test_frame_groups_equal_sessions(gctx, storage, graph):
         for grp in (0, 1):
             same(out[grp][k][0], x_ref, "frame %d (S=%d) group=%d" % (k, ns, grp))
             assert out[grp][k][1] == ll_ref
+
+
+def test_frame_groups_at_reference_geometry(gctx):
+    """The reference's compiled geometry (GFIRST.h:4-16: NRAD 147, two
+    z-sum radial blocks, odd-length interleaved rows; 168 angles, span 11):
+    four S=12 frames (a group of three and a lone frame) and two S=6 frames (a
+    group of two), FP16 storage, graph steps; each equals its own session bit
+    for bit, image and FP64 log-likelihood."""
+    g = REF
+    norm = norm_of(g, 13)
+    spec = [(1, 12)] * 4 + [(1, 6)] * 2
+    data = [frame_data(g, 700 + k, counts=2e6, bg=(k % 2 == 0)) for k in range(len(spec))]
+    bp = ga.batch_params(storage=ga.STORE_F16, max_frames=len(spec), min_nsub=6, lanes=2)
+    bt = gctx.batch(g, bp, norm)
+    for k, (ni, ns) in enumerate(spec):
+        assert bt.add(ni, ns, *data[k]) == k
+    bt.run()
+    got = [(bt.get(k), bt.loglik(k)) for k in range(len(spec))]
+    bt.close()
+    for k, (ni, ns) in enumerate(spec):
+        x_ref, ll_ref = session_result(gctx, g, norm, *data[k], ni, ns, ga.STORE_F16)
+        same(got[k][0], x_ref, "REF frame %d (S=%d)" % (k, ns))
+        assert got[k][1] == ll_ref
 
 
 def test_graph_replays_match_eager_each_step(gctx):
