@@ -49,15 +49,19 @@ __global__ __launch_bounds__(256) void k_rotate_tile(DevGeom G, const float* __r
                                                      int nl) {
   __shared__ float box[kRotTile * kRotStride];
   if (dsub) subset = *dsub;
+  // Logical order: bin tile, sample tile, angle, then plane chunk, so that
+  // every XCD (one contiguous range of logical blocks) rotates a few plane
+  // chunks for every angle and reads each image plane once into its L2.
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const unsigned nrt = (G.nrad + 31) / 32, nlt = (nl + 31) / 32;
   const unsigned nzc = (G.nzs + ZC - 1) / ZC;
+  const unsigned na = gridDim.x / (nrt * nlt * nzc);
   const unsigned rt = b % nrt;
   unsigned q = b / nrt;
   const unsigned lt = q % nlt;
   q /= nlt;
-  const int zc = (int)(q % nzc) * ZC;
-  const int a = (int)(q / nzc);
+  const int a = (int)(q % na);
+  const int zc = (int)(q / na) * ZC;
   if (a0 + a >= (G.nang - subset + nsub - 1) / nsub) return;  // whole block
   const int it = subset + (a0 + a) * nsub;
   const float st = G.sin_t[it], ct = G.cos_t[it];
