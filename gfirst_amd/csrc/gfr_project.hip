@@ -46,12 +46,15 @@ template <int ZC>
 __global__ __launch_bounds__(256) void k_rotate_tile(DevGeom G, const float* __restrict__ img,
                                                      float* __restrict__ R, int subset, int nsub,
                                                      const int* __restrict__ dsub, int a0,
-                                                     int nl) {
+                                                     int nl, int zouter) {
   __shared__ float box[kRotTile * kRotStride];
   if (dsub) subset = *dsub;
-  // Logical order: bin tile, sample tile, angle, then plane chunk, so that
-  // every XCD (one contiguous range of logical blocks) rotates a few plane
-  // chunks for every angle and reads each image plane once into its L2.
+  // Logical order: bin tile, sample tile, then angle and plane chunk; every
+  // XCD walks one contiguous range of logical blocks. With zouter the plane
+  // chunk is outermost: an XCD rotates a few plane chunks for every angle
+  // and reads each image plane once into its L2 (rotation outputs larger than
+  // an L2). Otherwise the angle is: an XCD writes the angles whose pairs its
+  // z-sum blocks read next, while they are still in its L2.
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const unsigned nrt = (G.nrad + 31) / 32, nlt = (nl + 31) / 32;
   const unsigned nzc = (G.nzs + ZC - 1) / ZC;
@@ -60,8 +63,8 @@ __global__ __launch_bounds__(256) void k_rotate_tile(DevGeom G, const float* __r
   unsigned q = b / nrt;
   const unsigned lt = q % nlt;
   q /= nlt;
-  const int a = (int)(q % na);
-  const int zc = (int)(q / na) * ZC;
+  const int a = (int)(zouter ? q % na : q / nzc);
+  const int zc = (int)(zouter ? q / na : q % nzc) * ZC;
   if (a0 + a >= (G.nang - subset + nsub - 1) / nsub) return;  // whole block
   const int it = subset + (a0 + a) * nsub;
   const float st = G.sin_t[it], ct = G.cos_t[it];
@@ -1126,15 +1129,18 @@ hipError_t launch_forward(const DevGeom& G, const float* img, float* R, size_t r
     const int na = min(chunk, nav - a0);
     const unsigned rbytes = (unsigned)((size_t)na * per * sizeof(float));
     const unsigned gxy = ((G.nrad + 31) / 32) * ((nl + 31) / 32) * na;
+    // plane chunks outermost when an angle pair's rotation exceeds an XCD's
+    // 4 MiB L2 (the z-sum cannot find it there anyway)
+    const int zouter = 2 * per * sizeof(float) > ((size_t)4 << 20);
     if (tune.rot_z == 4)
       hipLaunchKernelGGL(k_rotate_tile<4>, dim3(gxy * ((G.nzs + 3) / 4)), dim3(256), 0, st, G, img,
-                         R, subset, nsub, dsub, a0, nl);
+                         R, subset, nsub, dsub, a0, nl, zouter);
     else if (tune.rot_z == 32)
       hipLaunchKernelGGL(k_rotate_tile<32>, dim3(gxy * ((G.nzs + 31) / 32)), dim3(256), 0, st, G,
-                         img, R, subset, nsub, dsub, a0, nl);
+                         img, R, subset, nsub, dsub, a0, nl, zouter);
     else
       hipLaunchKernelGGL(k_rotate_tile<16>, dim3(gxy * ((G.nzs + 15) / 16)), dim3(256), 0, st, G,
-                         img, R, subset, nsub, dsub, a0, nl);
+                         img, R, subset, nsub, dsub, a0, nl, zouter);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     switch (tune.zsum_sg) {
