@@ -127,7 +127,10 @@ GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g) {
   memcpy(blob.data() + off_sino, h.sino.data(), n_sino * sizeof(int2));
   memcpy(blob.data() + off_zt, h.ztab.data(), h.ztab.size() * sizeof(uint2));
   hipError_t e = hipMalloc(&gd->block, total);
-  if (e == hipSuccess) e = hipMemcpy(gd->block, blob.data(), total, hipMemcpyHostToDevice);
+  // stream-ordered on the context's stream (nothing on the null stream)
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(gd->block, blob.data(), total, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     hip_err(ctx, e, "geometry tables");
     if (gd->block) (void)hipFree(gd->block);

@@ -490,7 +490,8 @@ int gfr_batch_add(gfr_batch* b, int niter, int nsub, const float* prompts, const
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return undo(hip_err(ctx, e, "gfr_batch_add"));
   unsigned int hbad = 0;
-  e = hipMemcpy(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost);
+  e = hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return undo(hip_err(ctx, e, "gfr_batch_add"));
   if (hbad) return undo(set_err(ctx, GFR_EINVAL, "non-finite value in the frame's sinograms"));
   // Tile shapes for this subset count, measured on the first lane's scratch.

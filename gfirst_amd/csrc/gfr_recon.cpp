@@ -198,12 +198,15 @@ int bg_present(gfr_ctx* ctx, const float* v, long long n, bool* out) {
     *out = any_positive(v, n);
     return GFR_OK;
   }
+  // stream-ordered on the context's stream (nothing on the null stream)
   unsigned int* flag = nullptr;
   unsigned int h = 0;
+  hipStream_t st = ctx->stream;
   hipError_t e = hipMalloc((void**)&flag, sizeof(unsigned int));
-  if (e == hipSuccess) e = hipMemcpy(flag, &h, sizeof(h), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_any_positive(v, n, flag, nullptr);
-  if (e == hipSuccess) e = hipMemcpy(&h, flag, sizeof(h), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = launch_zero(flag, sizeof(unsigned int), st);
+  if (e == hipSuccess) e = launch_any_positive(v, n, flag, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(h), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (flag) (void)hipFree(flag);
   if (e != hipSuccess) return hip_err(ctx, e, "gfr_recon_create background check");
   *out = h != 0;
@@ -381,7 +384,8 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   if (e == hipSuccess && r->ntypes == 2) e = upload(t2, r->yT);
   if (e == hipSuccess && hasBgT) e = upload(t2 + nd, r->bT);
   unsigned int hbad = 0;
-  if (e == hipSuccess) e = hipMemcpy(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e == hipSuccess && hbad) {
     (void)hipFree(stage);
     return rfail(r, set_err(ctx, GFR_EINVAL, "non-finite value in the measured sinograms"));
