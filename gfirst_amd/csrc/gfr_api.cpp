@@ -2,6 +2,7 @@
 // Each entry validates its arguments, resolves the geometry's device tables
 // (cached per context) and enqueues kernels on the given stream; nothing here
 // synchronises except the explicitly synchronous calls.
+#include <cstdlib>
 #include <stdio.h>
 #include <string.h>
 
@@ -164,6 +165,49 @@ GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g) {
   return gd;
 }
 
+hipError_t dev_alloc(gfr_ctx* ctx, void** p, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->kept.find(bytes);
+    if (it != ctx->kept.end()) {
+      *p = it->second;
+      ctx->kept.erase(it);
+      ctx->kept_bytes -= bytes;
+      return hipSuccess;
+    }
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    dev_trim(ctx);
+    e = hipMalloc(p, bytes);
+  }
+  return e;
+}
+
+void dev_free(gfr_ctx* ctx, void* p, size_t bytes) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->kept_bytes + bytes <= ctx->kept_max) {
+      ctx->kept.emplace(bytes, p);
+      ctx->kept_bytes += bytes;
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
+void dev_trim(gfr_ctx* ctx) {
+  std::multimap<size_t, void*> k;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    k.swap(ctx->kept);
+    ctx->kept_bytes = 0;
+  }
+  for (auto& kv : k) (void)hipFree(kv.second);
+}
+
 }  // namespace gfr
 
 using namespace gfr;
@@ -219,6 +263,8 @@ int gfr_ctx_create(int device, size_t hbm_budget, gfr_ctx** out) {
   gfr_ctx* c = new gfr_ctx();
   c->device = device;
   c->budget = hbm_budget;
+  c->kept_max = kKeptMax;
+  if (const char* env = getenv("GFR_KEPT_MAX_MB")) c->kept_max = (size_t)atoll(env) << 20;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return GFR_EHIP;
@@ -236,6 +282,7 @@ int gfr_ctx_destroy(gfr_ctx* ctx) {
   if (!ctx) return GFR_EINVAL;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  dev_trim(ctx);
   for (auto& kv : ctx->geoms) {
     if (kv.second->block) (void)hipFree(kv.second->block);
     if (kv.second->pb_cyl) (void)hipFree(kv.second->pb_cyl);

@@ -113,6 +113,10 @@ struct gfr_ctx {
   std::mutex err_mu;  // err is written by any thread using the context
   std::map<std::string, gfr::GeomDev*> geoms;
   std::map<void*, size_t> allocs;
+  // device blocks of finished sessions kept for reuse, by exact size
+  std::multimap<size_t, void*> kept;
+  size_t kept_bytes = 0;
+  size_t kept_max = 0;  // GFR_KEPT_MAX_MB (MiB), default kKeptMax
   unsigned int* d_scratch_u32 = nullptr;  // 256 words for small reductions
   gfr::Scratch rot;                       // rotation stage of per-operation forward calls
   gfr::Scratch pack;                      // packed ratio of per-operation backprojections
@@ -120,6 +124,17 @@ struct gfr_ctx {
 };
 
 namespace gfr {
+// Session memory: sessions of one request class (geometry and parameters)
+// allocate the same block sizes, so blocks of finished sessions are kept by
+// exact size (up to kKeptMax bytes) and handed to the next session instead
+// of a hipMalloc/hipFree pair per block (at most ctx->kept_max bytes kept:
+// environment GFR_KEPT_MAX_MB, default 32 GiB, 0 keeps nothing). dev_free
+// takes a block whose work is complete; dev_trim returns every kept block to
+// the device (also before a retried allocation that ran out of memory).
+constexpr size_t kKeptMax = (size_t)32 << 30;
+hipError_t dev_alloc(gfr_ctx* ctx, void** p, size_t bytes);
+void dev_free(gfr_ctx* ctx, void* p, size_t bytes);
+void dev_trim(gfr_ctx* ctx);
 // Returns scratch of at least `bytes`, ordered after its previous user.
 void* scratch_acquire(gfr_ctx* ctx, Scratch& s, size_t bytes, hipStream_t st, hipError_t* err);
 void scratch_release(Scratch& s, hipStream_t st);

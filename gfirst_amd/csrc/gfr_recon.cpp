@@ -59,7 +59,7 @@ struct gfr_recon {
   hipEvent_t ev_join = nullptr;
   bool tfree_pending = false;
   int llh_chain = 0;  // chain of the last pass's log-likelihood (0 doubles, 1 triples)
-  std::vector<void*> owned;
+  std::vector<std::pair<void*, size_t>> owned;  // blocks from dev_alloc
   size_t bytes = 0;
   Tune tune;
   int iter = 1, type = 0, subset = 0;
@@ -320,8 +320,9 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   if (hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking) != hipSuccess)
     return rfail(r, set_err(ctx, GFR_EHIP, "stream create"));
   auto A = [&](void** ptr, size_t b) -> bool {
-    if (hipMalloc(ptr, b ? b : 4) != hipSuccess) return false;
-    r->owned.push_back(*ptr);
+    const size_t nb = b ? b : 4;
+    if (dev_alloc(ctx, ptr, nb) != hipSuccess) return false;
+    r->owned.push_back({*ptr, nb});
     return true;
   };
   bool ok = A((void**)&r->xD, pl.img) && A((void**)&r->num, pl.img) && A((void**)&r->den, pl.img) &&
@@ -351,7 +352,7 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   hipError_t e = hipSuccess;
   // Host arrays reach the device through a transient staging buffer.
   float* stage = nullptr;
-  if (hipMalloc((void**)&stage, pl.sino_f32) != hipSuccess)
+  if (dev_alloc(ctx, (void**)&stage, pl.sino_f32) != hipSuccess)
     return rfail(r, set_err(ctx, GFR_ENOMEM, "reconstruction staging allocation failed"));
   // Measured sinograms with a NaN or infinite value fail the request: the
   // reference would carry it through the ratio into every voxel the LOR
@@ -387,7 +388,7 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e == hipSuccess && hbad) {
-    (void)hipFree(stage);
+    dev_free(ctx, stage, pl.sino_f32);  // stream synchronised above
     return rfail(r, set_err(ctx, GFR_EINVAL, "non-finite value in the measured sinograms"));
   }
   // Initial estimates (GFIRST.cu:282-288).
@@ -395,7 +396,8 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   if (e == hipSuccess && r->ntypes == 2)
     e = launch_fill_cylinder(G, p->mode == 1 ? 0.f : 1.f, r->xT, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  (void)hipFree(stage);
+  (void)hipStreamSynchronize(st);  // stage's work done before reuse
+  dev_free(ctx, stage, pl.sino_f32);
   if (e != hipSuccess) return rfail(r, hip_err(ctx, e, "gfr_recon_create"));
   if (p->mode == 3) {
     // BLANKT needs T x RES (GFIRST.cu:225-226, 246): scale a float copy.
@@ -651,7 +653,7 @@ int gfr_recon_destroy(gfr_recon* r) {
     (void)hipEventDestroy(pe.second.second);
   }
   for (hipEvent_t e : r->ev_free) (void)hipEventDestroy(e);
-  for (void* q : r->owned) (void)hipFree(q);
+  for (auto& q : r->owned) dev_free(r->ctx, q.first, q.second);  // streams synchronised above
   for (hipEvent_t e : {r->ev_snap, r->ev_tfree, r->ev_join})
     if (e) (void)hipEventDestroy(e);
   if (r->stT) (void)hipStreamDestroy(r->stT);

@@ -122,6 +122,27 @@ def test_inveon_size_requests_in_the_mix(gctx, orc):
     assert np.array_equal(got[1][0], ex)
 
 
+def test_kept_blocks_reused_by_later_sessions(gctx):
+    """Blocks of finished sessions are kept by size and handed to the next
+    session of the same class with the previous contents: a request run
+    again after the same request, after a joint request of another geometry
+    and after a doubles request with other data gives the same bits."""
+    reqs = [(C0, 1, 4, 2, ga.STORE_F16, 1e6), (SMALL, 3, 4, 1, ga.STORE_F32, 2e5)]
+    data = [request(g, 950 + k, mode, counts) for k, (g, mode, _, _, _, counts) in
+            enumerate(reqs)]
+    params = [ga.recon_params(mode=mode, niter=ni, nsub=S, storage=st)
+              for (_, mode, S, ni, st, _) in reqs]
+    first = [alone(gctx, g, p, *d) for (g, *_), p, d in zip(reqs, params, data)]
+    other = request(C0, 960, 1, 3e5)
+    for _ in range(2):
+        alone(gctx, C0, params[0], *other)
+        for k, ((g, *_), p, d) in enumerate(zip(reqs, params, data)):
+            xD, xT = alone(gctx, g, p, *d)
+            assert np.array_equal(xD, first[k][0]), k
+            if p.mode >= 2:
+                assert np.array_equal(xT, first[k][1]), k
+
+
 def test_admission_waits_for_budget_and_isolates_failures(gctx):
     g = C0
     p = ga.recon_params(mode=1, niter=1, nsub=4, storage=ga.STORE_F32)
