@@ -128,10 +128,7 @@ GeomDev* geom_for(gfr_ctx* ctx, const gfr_geometry* g) {
   memcpy(blob.data() + off_sino, h.sino.data(), n_sino * sizeof(int2));
   memcpy(blob.data() + off_zt, h.ztab.data(), h.ztab.size() * sizeof(uint2));
   hipError_t e = hipMalloc(&gd->block, total);
-  // stream-ordered on the context's stream (nothing on the null stream)
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(gd->block, blob.data(), total, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(gd->block, blob.data(), total, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     hip_err(ctx, e, "geometry tables");
     if (gd->block) (void)hipFree(gd->block);
@@ -264,7 +261,10 @@ int gfr_ctx_create(int device, size_t hbm_budget, gfr_ctx** out) {
   c->device = device;
   c->budget = hbm_budget;
   c->kept_max = kKeptMax;
-  if (const char* env = getenv("GFR_KEPT_MAX_MB")) c->kept_max = (size_t)atoll(env) << 20;
+  if (const char* env = getenv("GFR_KEPT_MAX_MB")) {
+    const long long mb = atoll(env);
+    c->kept_max = mb > 0 ? (size_t)mb << 20 : 0;
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return GFR_EHIP;

@@ -198,15 +198,12 @@ int bg_present(gfr_ctx* ctx, const float* v, long long n, bool* out) {
     *out = any_positive(v, n);
     return GFR_OK;
   }
-  // stream-ordered on the context's stream (nothing on the null stream)
   unsigned int* flag = nullptr;
   unsigned int h = 0;
-  hipStream_t st = ctx->stream;
   hipError_t e = hipMalloc((void**)&flag, sizeof(unsigned int));
-  if (e == hipSuccess) e = launch_zero(flag, sizeof(unsigned int), st);
-  if (e == hipSuccess) e = launch_any_positive(v, n, flag, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(h), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = hipMemcpy(flag, &h, sizeof(h), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_any_positive(v, n, flag, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(&h, flag, sizeof(h), hipMemcpyDeviceToHost);
   if (flag) (void)hipFree(flag);
   if (e != hipSuccess) return hip_err(ctx, e, "gfr_recon_create background check");
   *out = h != 0;
