@@ -205,6 +205,30 @@ void dev_trim(gfr_ctx* ctx) {
   for (auto& kv : k) (void)hipFree(kv.second);
 }
 
+hipError_t stream_take(gfr_ctx* ctx, hipStream_t* st) {
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (!ctx->kept_streams.empty()) {
+      *st = ctx->kept_streams.back();
+      ctx->kept_streams.pop_back();
+      return hipSuccess;
+    }
+  }
+  return hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+}
+
+void stream_give(gfr_ctx* ctx, hipStream_t st) {
+  if (!st) return;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->kept_max > 0 && ctx->kept_streams.size() < kKeptStreams) {
+      ctx->kept_streams.push_back(st);
+      return;
+    }
+  }
+  (void)hipStreamDestroy(st);
+}
+
 }  // namespace gfr
 
 using namespace gfr;
@@ -283,6 +307,7 @@ int gfr_ctx_destroy(gfr_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   dev_trim(ctx);
+  for (hipStream_t st : ctx->kept_streams) (void)hipStreamDestroy(st);
   for (auto& kv : ctx->geoms) {
     if (kv.second->block) (void)hipFree(kv.second->block);
     if (kv.second->pb_cyl) (void)hipFree(kv.second->pb_cyl);

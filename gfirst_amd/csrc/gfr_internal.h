@@ -117,6 +117,7 @@ struct gfr_ctx {
   std::multimap<size_t, void*> kept;
   size_t kept_bytes = 0;
   size_t kept_max = 0;  // GFR_KEPT_MAX_MB (MiB), default kKeptMax
+  std::vector<hipStream_t> kept_streams;  // idle non-blocking streams
   unsigned int* d_scratch_u32 = nullptr;  // 256 words for small reductions
   gfr::Scratch rot;                       // rotation stage of per-operation forward calls
   gfr::Scratch pack;                      // packed ratio of per-operation backprojections
@@ -135,6 +136,11 @@ constexpr size_t kKeptMax = (size_t)32 << 30;
 hipError_t dev_alloc(gfr_ctx* ctx, void** p, size_t bytes);
 void dev_free(gfr_ctx* ctx, void* p, size_t bytes);
 void dev_trim(gfr_ctx* ctx);
+// The same for session streams (non-blocking; at most kKeptStreams, none
+// when kept_max is 0): stream_give takes an idle stream.
+constexpr size_t kKeptStreams = 64;
+hipError_t stream_take(gfr_ctx* ctx, hipStream_t* st);
+void stream_give(gfr_ctx* ctx, hipStream_t st);
 // Returns scratch of at least `bytes`, ordered after its previous user.
 void* scratch_acquire(gfr_ctx* ctx, Scratch& s, size_t bytes, hipStream_t st, hipError_t* err);
 void scratch_release(Scratch& s, hipStream_t st);

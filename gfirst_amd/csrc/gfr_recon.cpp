@@ -314,7 +314,7 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
   r->gd = gd;
   r->bytes = need;
   r->ntypes = p->mode >= 2 ? 2 : 1;
-  if (hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking) != hipSuccess)
+  if (stream_take(ctx, &r->st) != hipSuccess)
     return rfail(r, set_err(ctx, GFR_EHIP, "stream create"));
   auto A = [&](void** ptr, size_t b) -> bool {
     const size_t nb = b ? b : 4;
@@ -337,7 +337,7 @@ int gfr_recon_create(gfr_ctx* ctx, const gfr_geometry* g, const gfr_recon_params
          A((void**)&r->corr2T, pl.corr2) && A((void**)&r->RT, pl.rot) && A((void**)&r->llh_partT, pl.llh) &&
          A((void**)&r->d_llhT, 64);
   if (ok && r->ntypes == 2)
-    ok = hipStreamCreateWithFlags(&r->stT, hipStreamNonBlocking) == hipSuccess &&
+    ok = stream_take(ctx, &r->stT) == hipSuccess &&
          hipEventCreateWithFlags(&r->ev_snap, hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&r->ev_tfree, hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) == hipSuccess;
@@ -653,8 +653,8 @@ int gfr_recon_destroy(gfr_recon* r) {
   for (auto& q : r->owned) dev_free(r->ctx, q.first, q.second);  // streams synchronised above
   for (hipEvent_t e : {r->ev_snap, r->ev_tfree, r->ev_join})
     if (e) (void)hipEventDestroy(e);
-  if (r->stT) (void)hipStreamDestroy(r->stT);
-  if (r->st) (void)hipStreamDestroy(r->st);
+  stream_give(r->ctx, r->stT);  // both synchronised above
+  stream_give(r->ctx, r->st);
   {
     std::lock_guard<std::mutex> lk(r->ctx->mu);
     r->ctx->in_use -= r->bytes;
