@@ -313,7 +313,9 @@ int gfr_recon_get(gfr_recon* r, int what, float* host_out);
 int gfr_recon_loglik(gfr_recon* r, double* out);
 /* The session's stream. In the joint modes the triples passes run on a
  * second stream of the session (beside the next iteration's doubles
- * passes) that joins this one when the last pass is enqueued. */
+ * passes) that joins this one when the last pass is enqueued. The context
+ * keeps the streams of destroyed sessions for later sessions: do not use it
+ * after gfr_recon_destroy. */
 void* gfr_recon_stream(gfr_recon* r);
 /* Waits for every pass enqueued so far (both streams of a joint session). */
 int gfr_recon_sync(gfr_recon* r);
